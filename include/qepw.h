@@ -1,0 +1,201 @@
+/*
+ * qepw.h -- C ABI of the gfx950 plane-wave h_psi / Davidson kernels.
+ *
+ * This library replaces, function for function, the native kernels that the
+ * reference (sjhong6230/QE_python, qepy-pw 1.0.2) binds from its Cython
+ * extension ``qepy_pw/_native_fft.pyx`` on the h_psi / Davidson path.
+ * Each entry point below cites the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every pointer named a "device" buffer is HIP device memory; host
+ *     buffers are named explicitly.  No entry point allocates device memory
+ *     except qepw_plan_create (its index tables).  Scratch is passed in as
+ *     ``workspace``; the matching *_workspace() call gives its size.
+ *   - Matrices are column-major (Fortran order) with a leading dimension, as
+ *     the reference's Davidson storage is.  Complex numbers are interleaved
+ *     (re, im) doubles (QEPW_C128) or floats (QEPW_C64).
+ *   - ``stream`` is a hipStream_t passed as void* (NULL = default stream).
+ *     Every call is asynchronous on that stream.
+ *   - Rows of all vectors are in the plan's DEVICE ORDER (stick-major, see
+ *     qepw_plan_permutation), except where a function says otherwise.
+ *   - Return value: QEPW_OK (0) or a negative error code; the message of the
+ *     last failure on the calling thread is given by qepw_last_error().  No
+ *     C++ exception crosses this boundary.
+ */
+#ifndef QEPW_H
+#define QEPW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QEPW_C128 0 /* complex128 storage and FP64 arithmetic */
+#define QEPW_C64 1  /* complex64 storage and FP32 arithmetic  */
+
+#define QEPW_OP_N 0 /* op(A) = A   */
+#define QEPW_OP_C 1 /* op(A) = A^H */
+
+#define QEPW_OK 0
+#define QEPW_ERR_ARG (-1)         /* invalid argument (ValueError)            */
+#define QEPW_ERR_HIP (-2)         /* HIP runtime failure (RuntimeError)       */
+#define QEPW_ERR_UNSUPPORTED (-3) /* shape beyond the kernels' limits         */
+#define QEPW_ERR_WORKSPACE (-4)   /* workspace smaller than required          */
+
+typedef struct qepw_plan qepw_plan;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* qepw_last_error(void);
+/* ABI version, currently 1. */
+int qepw_abi_version(void);
+
+/*
+ * Plan of the local-potential FFT for one k-point basis.
+ * Replaces: LocalPotentialWorkspace.__init__, serial branch
+ *   (qepy_pw/basis.py:460-555: linear_slots, z_major_slots,
+ *   serial_stick_positions) together with NativeFFTWPlan
+ *   (qepy_pw/_native_fft.pyx:1973-2086).
+ * miller: host int32 (npw, 3) C-order Miller indices in reference row order;
+ * slot of row g = (m0 mod n1, m1 mod n2, m2 mod n3).  ``device`` is the HIP
+ * device that holds the plan's tables.
+ */
+int qepw_plan_create(const int32_t* miller, int64_t npw, int n1, int n2, int n3,
+                     int device, qepw_plan** out);
+int qepw_plan_destroy(qepw_plan* plan);
+/* info[0..7] = npw, n1, n2, n3, active sticks, active x columns,
+ *              sticks per z-stage workgroup, full-grid sticks. */
+int qepw_plan_query(const qepw_plan* plan, int64_t* info);
+/* perm[d] (host int32, npw) = reference row of device row d. */
+int qepw_plan_permutation(const qepw_plan* plan, int32_t* perm);
+/* Device copy of the same permutation (int32, npw). */
+const int32_t* qepw_plan_device_permutation(const qepw_plan* plan);
+
+/*
+ * result[:, b] = FFT^-1-sandwich(V(r) . FFT(vectors[:, b])) / N
+ *                + kinetic . vectors[:, b]          (b < nb)
+ * Replaces: apply_serial (qepy_pw/_native_fft.pyx:2338-2455) and
+ *   apply_serial_spatial (:2458-2534, C :58-135), called from
+ *   LocalPotentialWorkspace.apply (qepy_pw/basis.py:1354-1455).
+ * potential: device real (n3, n1, n2) plane-major grid (the reference's
+ *   "native" layout), double for QEPW_C128, float for QEPW_C64.
+ * kinetic: device real (npw) in device order, same type, or NULL.
+ * vectors / result: device complex, column-major, leading dims ldv / ldr.
+ * workspace: at least qepw_apply_local_workspace(plan, precision, 1) bytes;
+ *   larger workspaces process more bands per launch.
+ */
+size_t qepw_apply_local_workspace(const qepw_plan* plan, int precision, int nb);
+int qepw_apply_local(const qepw_plan* plan, int precision, const void* vectors,
+                     int64_t ldv, int nb, const void* potential, const void* kinetic,
+                     void* result, int64_t ldr, void* workspace, size_t ws_bytes,
+                     void* stream);
+
+/*
+ * out[z, x, y] = Re sum_G V(G) exp(+i G.r) (unnormalised backward FFT) of a
+ * C-order (n1, n2, n3) complex128 grid, written in (n3, n1, n2) layout.
+ * Replaces: inverse_real (qepy_pw/_native_fft.pyx:2188-2202) as used by
+ *   LocalPotentialWorkspace.prepare_potential (qepy_pw/basis.py:777-792).
+ */
+size_t qepw_inverse_real_workspace(const qepw_plan* plan);
+int qepw_inverse_real(const qepw_plan* plan, const void* grid_g, double* out,
+                      void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * C (m x n) = alpha op(A) B + beta C, complex, column-major, on MFMA
+ * (v_mfma_f64_16x16x4f64 for QEPW_C128, v_mfma_f32_16x16x4f32 for QEPW_C64).
+ * op_a = QEPW_OP_N: A is m x k;  QEPW_OP_C: A is k x m and op(A) = A^H.
+ * alpha, beta: host (re, im) pairs.  A^H B with large k is split over k with
+ * a fixed-order (deterministic) reduction through ``workspace``.
+ * Replaces: the zgemm bridge qepy_zgemm64 (qepy_pw/_native_fft.pyx:595-634)
+ *   and its users projector_overlaps (:694-731, becp = beta^H psi) and
+ *   add_projector_product (:782-836, Hpsi += beta (D becp)), plus the
+ *   NumPy products of davidson's update_projected / ritz_pairs
+ *   (qepy_pw/diagonalization.py:658-726) and _matmul_f (:74-80).
+ */
+size_t qepw_gemm_workspace(int precision, int op_a, int64_t m, int64_t n, int64_t k);
+int qepw_gemm(int precision, int op_a, int64_t m, int64_t n, int64_t k,
+              const double* alpha, const void* a, int64_t lda, const void* b,
+              int64_t ldb, const double* beta, void* c, int64_t ldc, void* workspace,
+              size_t ws_bytes, void* stream);
+
+/*
+ * out[b] = sum_g |v[g, b]|^2 (device double, FP64 accumulation for both
+ * precisions, fixed reduction order).
+ * Replaces: column_squared_norms (qepy_pw/_native_fft.pyx:1153-1191).
+ */
+int qepw_column_squared_norms(int precision, const void* v, int64_t ldv, int64_t nrows,
+                              int nb, double* out, void* stream);
+
+/*
+ * out[b] = sum_g conj(l[g, b]) r[g, b]  (device, nb (re, im) double pairs).
+ * Replaces: column_inner_products (qepy_pw/_native_fft.pyx:1241-1291).
+ */
+int qepw_column_inner_products(int precision, const void* l, int64_t ldl, const void* r,
+                               int64_t ldr, int64_t nrows, int nb, double* out,
+                               void* stream);
+
+/*
+ * r[:, b] -= e[b] x[:, b]  (e: device double).
+ * Replaces: subtract_band_energies (qepy_pw/_native_fft.pyx:1194-1238).
+ */
+int qepw_subtract_band_energies(int precision, void* r, int64_t ldr, const void* x,
+                                int64_t ldx, const double* e, int64_t nrows, int nb,
+                                void* stream);
+
+/*
+ * QE g_psi: for j < nb, with c = cols[j] (or j when cols is NULL):
+ *   x = 2 (diag[g] - e[j]);  d = (1 + x + sqrt(1 + (x-1)^2)) / 2;
+ *   out[g, j] = 2 r[g, c] / d.
+ * e, diag: device double; cols: device int32 or NULL.
+ * Replaces: qe_precondition (qepy_pw/_native_fft.pyx:852-899) as called by
+ *   _qe_precondition (qepy_pw/diagonalization.py:475-497, 771-780).
+ */
+int qepw_qe_precondition(int precision, const void* r, int64_t ldr, const int32_t* cols,
+                         const double* e, const double* diag, void* out, int64_t ldo,
+                         int64_t nrows, int nb, void* stream);
+
+/*
+ * dst[:, j] = src[:, sel[j]] / sqrt(sq[sel[j]])  (sq: device double,
+ * sel: device int32).
+ * Replaces: normalize_selected_columns (qepy_pw/_native_fft.pyx:1495-1530).
+ */
+int qepw_normalize_selected_columns(int precision, const void* src, int64_t lds,
+                                    const double* sq, const int32_t* sel, void* dst,
+                                    int64_t ldd, int64_t nrows, int nb, void* stream);
+
+/*
+ * hv[:, b] = x[:, b] e[b] + r[:, b]: H psi of the Ritz vectors rebuilt from
+ * the residual at a Davidson restart, without an extra H application.
+ * Replaces: the restart block of davidson (qepy_pw/diagonalization.py:843-868).
+ */
+int qepw_restart_applied(int precision, const void* x, int64_t ldx, const void* r,
+                         int64_t ldr, const double* e, void* hv, int64_t ldh,
+                         int64_t nrows, int nb, void* stream);
+
+/*
+ * Row permutation with optional precision change between the reference row
+ * order and the device order.  direction 0 (gather): dst[d] = src[perm[d]];
+ * direction 1 (scatter): dst[perm[d]] = src[d].  perm: device int32.
+ * No reference counterpart: it is the device layout change at the boundary
+ * (the reference indexes rows through linear_slots, basis.py:515-523).
+ */
+int qepw_permute_rows(int src_precision, int dst_precision, int direction,
+                      const int32_t* perm, const void* src, int64_t lds, void* dst,
+                      int64_t ldd, int64_t nrows, int nb, void* stream);
+
+/*
+ * diagonal[g] += multiplicity * Re sum_i coupled[g, i] conj(beta[g, i])
+ * (complex128 inputs, device double output).
+ * Replaces: accumulate_projector_diagonal (qepy_pw/_native_fft.pyx:734-779).
+ */
+int qepw_accumulate_projector_diagonal(const void* coupled, int64_t ldc, const void* beta,
+                                       int64_t ldb, int64_t nrows, int nch,
+                                       double multiplicity, double* diagonal,
+                                       void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QEPW_H */

@@ -1,0 +1,105 @@
+"""ctypes binding of ``libqepw_hip.so`` (the C ABI of ``include/qepw.h``).
+
+This is the device counterpart of the reference's ``_load_native_fft()``
+(``qepy_pw/basis.py:48-74``): it loads the in-tree gfx950 library once and
+raises if it is missing -- there is no CPU fallback.  Error codes returned
+by the library become the reference's exception types: ``ValueError`` for
+invalid arguments, ``RuntimeError`` for HIP / unsupported-shape failures.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import c_double, c_int, c_int64, c_size_t, c_void_p, POINTER
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libqepw_hip.so"
+
+C128 = 0
+C64 = 1
+OP_N = 0
+OP_C = 1
+
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_UNSUPPORTED = -3
+ERR_WORKSPACE = -4
+
+# (name, restype, argtypes) of every exported symbol, in header order.
+SIGNATURES = [
+    ("qepw_last_error", ctypes.c_char_p, []),
+    ("qepw_abi_version", c_int, []),
+    ("qepw_plan_create", c_int, [c_void_p, c_int64, c_int, c_int, c_int, c_int, POINTER(c_void_p)]),
+    ("qepw_plan_destroy", c_int, [c_void_p]),
+    ("qepw_plan_query", c_int, [c_void_p, POINTER(c_int64)]),
+    ("qepw_plan_permutation", c_int, [c_void_p, c_void_p]),
+    ("qepw_plan_device_permutation", c_void_p, [c_void_p]),
+    ("qepw_apply_local_workspace", c_size_t, [c_void_p, c_int, c_int]),
+    ("qepw_apply_local", c_int, [c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p, c_void_p,
+                                 c_void_p, c_int64, c_void_p, c_size_t, c_void_p]),
+    ("qepw_inverse_real_workspace", c_size_t, [c_void_p]),
+    ("qepw_inverse_real", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    ("qepw_gemm_workspace", c_size_t, [c_int, c_int, c_int64, c_int64, c_int64]),
+    ("qepw_gemm", c_int, [c_int, c_int, c_int64, c_int64, c_int64, POINTER(c_double), c_void_p,
+                          c_int64, c_void_p, c_int64, POINTER(c_double), c_void_p, c_int64,
+                          c_void_p, c_size_t, c_void_p]),
+    ("qepw_column_squared_norms", c_int, [c_int, c_void_p, c_int64, c_int64, c_int, c_void_p,
+                                          c_void_p]),
+    ("qepw_column_inner_products", c_int, [c_int, c_void_p, c_int64, c_void_p, c_int64, c_int64,
+                                           c_int, c_void_p, c_void_p]),
+    ("qepw_subtract_band_energies", c_int, [c_int, c_void_p, c_int64, c_void_p, c_int64,
+                                            c_void_p, c_int64, c_int, c_void_p]),
+    ("qepw_qe_precondition", c_int, [c_int, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_int64, c_int64, c_int, c_void_p]),
+    ("qepw_normalize_selected_columns", c_int, [c_int, c_void_p, c_int64, c_void_p, c_void_p,
+                                                c_void_p, c_int64, c_int64, c_int, c_void_p]),
+    ("qepw_restart_applied", c_int, [c_int, c_void_p, c_int64, c_void_p, c_int64, c_void_p,
+                                     c_void_p, c_int64, c_int64, c_int, c_void_p]),
+    ("qepw_permute_rows", c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p,
+                                  c_int64, c_int64, c_int, c_void_p]),
+    ("qepw_accumulate_projector_diagonal", c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int64,
+                                                   c_int, c_double, c_void_p, c_void_p]),
+]
+
+_LIB: ctypes.CDLL | None = None
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    """Load the gfx950 kernel library (once); raise if it was not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise NativeLibraryMissing(
+            f"{LIB_PATH} is missing: build it with `python -m qe_python_amd.build` "
+            "(the device path has no CPU fallback)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, restype, argtypes in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _LIB = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().qepw_last_error().decode(errors="replace")
+
+
+def check(rc: int, what: str) -> None:
+    if rc == OK:
+        return
+    msg = last_error() or what
+    if rc == ERR_ARG:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def complex_pair(z) -> ctypes.Array:
+    z = complex(z)
+    return (c_double * 2)(z.real, z.imag)

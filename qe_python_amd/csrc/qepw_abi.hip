@@ -1,0 +1,399 @@
+// extern "C" boundary of the library (declared in include/qepw.h): argument
+// checks, plan construction and error reporting.  No C++ exception crosses
+// this boundary; failures return a negative code and set a per-thread
+// message.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "qepw_internal.h"
+
+struct qepw_plan {
+  qepw::Plan p;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char* msg) {
+  g_error = msg;
+  return code;
+}
+
+int fail_hip(hipError_t e, const char* where) {
+  g_error = std::string(where) + ": " + hipGetErrorString(e);
+  return QEPW_ERR_HIP;
+}
+
+int finish(int rc, const char* where) {
+  if (rc == QEPW_OK) return rc;
+  switch (rc) {
+    case QEPW_ERR_ARG: g_error = std::string(where) + ": invalid argument"; break;
+    case QEPW_ERR_UNSUPPORTED:
+      g_error = std::string(where) + ": shape outside the kernels' supported range";
+      break;
+    case QEPW_ERR_WORKSPACE: g_error = std::string(where) + ": workspace too small"; break;
+    default: {
+      hipError_t e = hipGetLastError();
+      g_error = std::string(where) + ": HIP failure (" + hipGetErrorString(e) + ")";
+    }
+  }
+  return rc;
+}
+
+bool line_plan(int n, qepw::LinePlan* lp) {
+  if (n < 1 || n > qepw::kMaxLine) return false;
+  lp->n = n;
+  lp->nstages = 0;
+  int m = n;
+  const int radices[4] = {4, 2, 3, 5};
+  for (int r : radices) {
+    while (m % r == 0) {
+      if (lp->nstages >= qepw::kMaxStages) return false;
+      lp->radix[lp->nstages++] = r;
+      m /= r;
+    }
+  }
+  for (int s = lp->nstages; s < qepw::kMaxStages; ++s) lp->radix[s] = 1;
+  return m == 1;
+}
+
+template <typename T>
+hipError_t upload(T** dst, const std::vector<T>& src) {
+  *dst = nullptr;
+  if (src.empty()) return hipSuccess;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), src.size() * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+void free_tables(qepw::StickTables& t) {
+  hipFree(t.d_off);
+  hipFree(t.d_gpos);
+  hipFree(t.d_plane);
+  hipFree(t.d_xcols);
+  t.d_off = t.d_gpos = t.d_plane = t.d_xcols = nullptr;
+}
+
+void free_plan(qepw::Plan& p) {
+  free_tables(p.sphere);
+  free_tables(p.full);
+  for (int a = 0; a < 3; ++a) {
+    hipFree(p.d_tw64[a]);
+    hipFree(p.d_tw32[a]);
+    p.d_tw64[a] = nullptr;
+    p.d_tw32[a] = nullptr;
+  }
+  hipFree(p.d_perm);
+  p.d_perm = nullptr;
+}
+
+struct TableHost {
+  std::vector<int> off, gpos, plane, xcols;
+};
+
+hipError_t upload_tables(qepw::StickTables& t, const TableHost& h) {
+  hipError_t e;
+  if ((e = upload(&t.d_off, h.off)) != hipSuccess) return e;
+  if ((e = upload(&t.d_gpos, h.gpos)) != hipSuccess) return e;
+  if ((e = upload(&t.d_plane, h.plane)) != hipSuccess) return e;
+  return upload(&t.d_xcols, h.xcols);
+}
+
+int pmod(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* qepw_last_error(void) { return g_error.c_str(); }
+
+int qepw_abi_version(void) { return 1; }
+
+int qepw_plan_create(const int32_t* miller, int64_t npw, int n1, int n2, int n3, int device,
+                     qepw_plan** out) {
+  if (!out) return fail(QEPW_ERR_ARG, "qepw_plan_create: out is NULL");
+  *out = nullptr;
+  if (!miller || npw < 1) return fail(QEPW_ERR_ARG, "qepw_plan_create: empty basis");
+  if (npw > 0x7fffffff) return fail(QEPW_ERR_UNSUPPORTED, "qepw_plan_create: npw too large");
+  qepw::LinePlan lp[3];
+  const int dims[3] = {n1, n2, n3};
+  for (int a = 0; a < 3; ++a)
+    if (!line_plan(dims[a], &lp[a]))
+      return fail(QEPW_ERR_UNSUPPORTED,
+                  "qepw_plan_create: FFT lengths must be 2,3,5-smooth and <= 256");
+  const int64_t ngrid = (int64_t)n1 * n2 * n3;
+  if (ngrid > 0x7fffffff) return fail(QEPW_ERR_UNSUPPORTED, "qepw_plan_create: grid too large");
+
+  // slots and stick-major order
+  std::vector<int64_t> key(npw);
+  std::vector<int> zs(npw), stick_key(npw);
+  for (int64_t g = 0; g < npw; ++g) {
+    const int x = pmod(miller[3 * g], n1), y = pmod(miller[3 * g + 1], n2),
+              z = pmod(miller[3 * g + 2], n3);
+    stick_key[g] = x * n2 + y;
+    zs[g] = z;
+    key[g] = (int64_t)stick_key[g] * n3 + z;
+  }
+  std::vector<int> perm(npw);
+  for (int64_t g = 0; g < npw; ++g) perm[g] = (int)g;
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  for (int64_t d = 1; d < npw; ++d)
+    if (key[perm[d]] == key[perm[d - 1]])
+      return fail(QEPW_ERR_ARG, "qepw_plan_create: two plane waves share one FFT slot "
+                                "(grid too small for the basis)");
+
+  TableHost sphere;
+  std::vector<int> xs_seen(n1, 0);
+  sphere.gpos.resize(npw);
+  int nst = 0;
+  for (int64_t d = 0; d < npw; ++d) {
+    const int g = perm[d];
+    if (d == 0 || stick_key[g] != stick_key[perm[d - 1]]) {
+      sphere.off.push_back((int)d);
+      sphere.plane.push_back(stick_key[g]);
+      xs_seen[stick_key[g] / n2] = 1;
+      ++nst;
+    }
+    sphere.gpos[d] = (nst - 1) * n3 + zs[g];
+  }
+  sphere.off.push_back((int)npw);
+  for (int x = 0; x < n1; ++x)
+    if (xs_seen[x]) sphere.xcols.push_back(x);
+
+  TableHost full;
+  const int nfull = n1 * n2;
+  full.off.resize(nfull + 1);
+  full.plane.resize(nfull);
+  full.gpos.resize(ngrid);
+  for (int s = 0; s <= nfull; ++s) full.off[s] = s * n3;
+  for (int s = 0; s < nfull; ++s) full.plane[s] = s;
+  for (int64_t i = 0; i < ngrid; ++i) full.gpos[i] = (int)i;
+  for (int x = 0; x < n1; ++x) full.xcols.push_back(x);
+
+  qepw_plan* plan = new (std::nothrow) qepw_plan();
+  if (!plan) return fail(QEPW_ERR_HIP, "qepw_plan_create: out of host memory");
+  qepw::Plan& p = plan->p;
+  p.n1 = n1;
+  p.n2 = n2;
+  p.n3 = n3;
+  p.npw = npw;
+  p.device = device;
+  for (int a = 0; a < 3; ++a) p.line[a] = lp[a];
+  p.sphere.nst = nst;
+  p.sphere.nxcols = (int)sphere.xcols.size();
+  p.sphere.npoints = npw;
+  p.sphere.group = qepw::stick_group_for(n3, nst);
+  p.sphere.ngroups = (nst + p.sphere.group - 1) / p.sphere.group;
+  p.full.nst = nfull;
+  p.full.nxcols = n1;
+  p.full.npoints = ngrid;
+  p.full.group = qepw::stick_group_for(n3, nfull);
+  p.full.ngroups = (nfull + p.full.group - 1) / p.full.group;
+  p.h_perm = perm;
+
+  int prev = 0;
+  hipGetDevice(&prev);
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = upload_tables(p.sphere, sphere);
+  if (e == hipSuccess) e = upload_tables(p.full, full);
+  if (e == hipSuccess) e = upload(&p.d_perm, perm);
+  for (int a = 0; a < 3 && e == hipSuccess; ++a) {
+    const int n = dims[a];
+    std::vector<double2> t64(n);
+    std::vector<float2> t32(n);
+    for (int k = 0; k < n; ++k) {
+      const long double ang = -2.0L * 3.141592653589793238462643383279502884L * k / n;
+      t64[k].x = (double)cosl(ang);
+      t64[k].y = (double)sinl(ang);
+      t32[k].x = (float)cosl(ang);
+      t32[k].y = (float)sinl(ang);
+    }
+    e = upload(&p.d_tw64[a], t64);
+    if (e == hipSuccess) e = upload(&p.d_tw32[a], t32);
+  }
+  hipSetDevice(prev);
+  if (e != hipSuccess) {
+    free_plan(p);
+    delete plan;
+    return fail_hip(e, "qepw_plan_create");
+  }
+  *out = plan;
+  return QEPW_OK;
+}
+
+int qepw_plan_destroy(qepw_plan* plan) {
+  if (!plan) return QEPW_OK;
+  int prev = 0;
+  hipGetDevice(&prev);
+  hipSetDevice(plan->p.device);
+  free_plan(plan->p);
+  hipSetDevice(prev);
+  delete plan;
+  return QEPW_OK;
+}
+
+int qepw_plan_query(const qepw_plan* plan, int64_t* info) {
+  if (!plan || !info) return fail(QEPW_ERR_ARG, "qepw_plan_query: NULL argument");
+  const qepw::Plan& p = plan->p;
+  info[0] = p.npw;
+  info[1] = p.n1;
+  info[2] = p.n2;
+  info[3] = p.n3;
+  info[4] = p.sphere.nst;
+  info[5] = p.sphere.nxcols;
+  info[6] = p.sphere.group;
+  info[7] = p.full.nst;
+  return QEPW_OK;
+}
+
+int qepw_plan_permutation(const qepw_plan* plan, int32_t* perm) {
+  if (!plan || !perm) return fail(QEPW_ERR_ARG, "qepw_plan_permutation: NULL argument");
+  std::memcpy(perm, plan->p.h_perm.data(), plan->p.h_perm.size() * sizeof(int32_t));
+  return QEPW_OK;
+}
+
+const int32_t* qepw_plan_device_permutation(const qepw_plan* plan) {
+  return plan ? plan->p.d_perm : nullptr;
+}
+
+size_t qepw_apply_local_workspace(const qepw_plan* plan, int precision, int nb) {
+  if (!plan || nb < 0 || (precision != QEPW_C128 && precision != QEPW_C64)) return 0;
+  return qepw::apply_local_band_bytes(plan->p, precision) * (size_t)nb;
+}
+
+int qepw_apply_local(const qepw_plan* plan, int precision, const void* vectors, int64_t ldv,
+                     int nb, const void* potential, const void* kinetic, void* result,
+                     int64_t ldr, void* workspace, size_t ws_bytes, void* stream) {
+  if (!plan) return fail(QEPW_ERR_ARG, "qepw_apply_local: NULL plan");
+  if (nb < 0) return fail(QEPW_ERR_ARG, "qepw_apply_local: negative band count");
+  if (nb == 0) return QEPW_OK;
+  if (!vectors || !potential || !result || !workspace)
+    return fail(QEPW_ERR_ARG, "qepw_apply_local: NULL buffer");
+  if (ldv < plan->p.npw || ldr < plan->p.npw)
+    return fail(QEPW_ERR_ARG, "qepw_apply_local: leading dimension smaller than npw");
+  if (vectors == result) return fail(QEPW_ERR_ARG, "qepw_apply_local: result aliases input");
+  return finish(qepw::apply_local(plan->p, precision, vectors, ldv, nb, potential, kinetic,
+                                  result, ldr, workspace, ws_bytes,
+                                  static_cast<hipStream_t>(stream)),
+                "qepw_apply_local");
+}
+
+size_t qepw_inverse_real_workspace(const qepw_plan* plan) {
+  return plan ? qepw::inverse_real_bytes(plan->p) : 0;
+}
+
+int qepw_inverse_real(const qepw_plan* plan, const void* grid_g, double* out, void* workspace,
+                      size_t ws_bytes, void* stream) {
+  if (!plan || !grid_g || !out || !workspace)
+    return fail(QEPW_ERR_ARG, "qepw_inverse_real: NULL argument");
+  return finish(qepw::inverse_real(plan->p, grid_g, out, workspace, ws_bytes,
+                                   static_cast<hipStream_t>(stream)),
+                "qepw_inverse_real");
+}
+
+size_t qepw_gemm_workspace(int precision, int op_a, int64_t m, int64_t n, int64_t k) {
+  return qepw::gemm_workspace_bytes(precision, op_a, m, n, k);
+}
+
+int qepw_gemm(int precision, int op_a, int64_t m, int64_t n, int64_t k, const double* alpha,
+              const void* a, int64_t lda, const void* b, int64_t ldb, const double* beta,
+              void* c, int64_t ldc, void* workspace, size_t ws_bytes, void* stream) {
+  if (!alpha || !beta) return fail(QEPW_ERR_ARG, "qepw_gemm: NULL alpha/beta");
+  if ((m > 0 && n > 0) && (!c || (k > 0 && (!a || !b))))
+    return fail(QEPW_ERR_ARG, "qepw_gemm: NULL matrix");
+  return finish(qepw::gemm(precision, op_a, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                           workspace, ws_bytes, static_cast<hipStream_t>(stream)),
+                "qepw_gemm");
+}
+
+int qepw_column_squared_norms(int precision, const void* v, int64_t ldv, int64_t nrows, int nb,
+                              double* out, void* stream) {
+  if (nb > 0 && (!v || !out || ldv < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_column_squared_norms: bad argument");
+  return finish(qepw::column_squared_norms(precision, v, ldv, nrows, nb, out,
+                                           static_cast<hipStream_t>(stream)),
+                "qepw_column_squared_norms");
+}
+
+int qepw_column_inner_products(int precision, const void* l, int64_t ldl, const void* r,
+                               int64_t ldr, int64_t nrows, int nb, double* out,
+                               void* stream) {
+  if (nb > 0 && (!l || !r || !out || ldl < nrows || ldr < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_column_inner_products: bad argument");
+  return finish(qepw::column_inner_products(precision, l, ldl, r, ldr, nrows, nb, out,
+                                            static_cast<hipStream_t>(stream)),
+                "qepw_column_inner_products");
+}
+
+int qepw_subtract_band_energies(int precision, void* r, int64_t ldr, const void* x,
+                                int64_t ldx, const double* e, int64_t nrows, int nb,
+                                void* stream) {
+  if (nb > 0 && (!r || !x || !e || ldr < nrows || ldx < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_subtract_band_energies: bad argument");
+  return finish(qepw::subtract_band_energies(precision, r, ldr, x, ldx, e, nrows, nb,
+                                             static_cast<hipStream_t>(stream)),
+                "qepw_subtract_band_energies");
+}
+
+int qepw_qe_precondition(int precision, const void* r, int64_t ldr, const int32_t* cols,
+                         const double* e, const double* diag, void* out, int64_t ldo,
+                         int64_t nrows, int nb, void* stream) {
+  if (nb > 0 && (!r || !e || !diag || !out || ldr < nrows || ldo < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_qe_precondition: bad argument");
+  return finish(qepw::qe_precondition(precision, r, ldr, cols, e, diag, out, ldo, nrows, nb,
+                                      static_cast<hipStream_t>(stream)),
+                "qepw_qe_precondition");
+}
+
+int qepw_normalize_selected_columns(int precision, const void* src, int64_t lds,
+                                    const double* sq, const int32_t* sel, void* dst,
+                                    int64_t ldd, int64_t nrows, int nb, void* stream) {
+  if (nb > 0 && (!src || !sq || !sel || !dst || lds < nrows || ldd < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_normalize_selected_columns: bad argument");
+  return finish(qepw::normalize_selected_columns(precision, src, lds, sq, sel, dst, ldd, nrows,
+                                                 nb, static_cast<hipStream_t>(stream)),
+                "qepw_normalize_selected_columns");
+}
+
+int qepw_restart_applied(int precision, const void* x, int64_t ldx, const void* r,
+                         int64_t ldr, const double* e, void* hv, int64_t ldh, int64_t nrows,
+                         int nb, void* stream) {
+  if (nb > 0 && (!x || !r || !e || !hv || ldx < nrows || ldr < nrows || ldh < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_restart_applied: bad argument");
+  return finish(qepw::restart_applied(precision, x, ldx, r, ldr, e, hv, ldh, nrows, nb,
+                                      static_cast<hipStream_t>(stream)),
+                "qepw_restart_applied");
+}
+
+int qepw_permute_rows(int src_precision, int dst_precision, int direction, const int32_t* perm,
+                      const void* src, int64_t lds, void* dst, int64_t ldd, int64_t nrows,
+                      int nb, void* stream) {
+  if (nb > 0 && (!src || !dst || lds < nrows || ldd < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_permute_rows: bad argument");
+  if (src == dst && perm) return fail(QEPW_ERR_ARG, "qepw_permute_rows: in-place permutation");
+  return finish(qepw::permute_rows(src_precision, dst_precision, direction, perm, src, lds, dst,
+                                   ldd, nrows, nb, static_cast<hipStream_t>(stream)),
+                "qepw_permute_rows");
+}
+
+int qepw_accumulate_projector_diagonal(const void* coupled, int64_t ldc, const void* beta,
+                                       int64_t ldb, int64_t nrows, int nch,
+                                       double multiplicity, double* diagonal, void* stream) {
+  if (nrows > 0 && nch > 0 && (!coupled || !beta || !diagonal || ldc < nrows || ldb < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_accumulate_projector_diagonal: bad argument");
+  if (nch <= 0) return QEPW_OK;
+  return finish(qepw::row_real_dot(coupled, ldc, beta, ldb, nrows, nch, multiplicity, diagonal,
+                                   static_cast<hipStream_t>(stream)),
+                "qepw_accumulate_projector_diagonal");
+}
+
+}  // extern "C"

@@ -1,0 +1,252 @@
+// Sphere-pruned local-potential pipeline: V_loc(r) psi + T psi on gfx950.
+//
+// Per band the reference computes (qepy_pw/_native_fft.pyx:2338-2455,
+// apply_serial; stick/plane form at :58-135, apply_serial_spatial):
+//   grid = 0; grid[slot_G] = psi_G; backward 3-D FFT; grid *= V(r);
+//   forward 3-D FFT; out_G = grid[slot_G] / N + T_G psi_G.
+// Here the 3-D FFT is split into three launches over all bands at once:
+//   A  k_sticks_inverse : gather the rows of ``group`` sticks (contiguous in
+//                         the device order) into LDS lines, backward z-FFT,
+//                         write the (n3, nst) stick buffer;
+//   B  k_planes         : one workgroup per (z-plane, band): load the plane's
+//                         active sticks into an n1 x n2 LDS plane, backward
+//                         y-FFT on active x columns only, backward x-FFT,
+//                         multiply by V(z, x, y), forward x-FFT, forward
+//                         y-FFT on active columns, store the active sticks;
+//   C  k_sticks_forward : forward z-FFT, gather the sphere rows, scale 1/N and
+//                         add T_G psi_G.
+// Only active sticks (19% of the 72^3 grid at 30 Ry) travel through HBM.
+#include "qepw_internal.h"
+#include "qepw_fft_lds.h"
+
+namespace qepw {
+
+constexpr int kStickPts = 16;
+
+template <typename T>
+__global__ __launch_bounds__(kStickThreads) void k_sticks_inverse(
+    const cplx<T>* __restrict__ psi, int64_t ldpsi, cplx<T>* __restrict__ work,
+    int64_t band_stride, int nst, int n3, const int* __restrict__ off,
+    const int* __restrict__ gpos, int group, LinePlan pz,
+    const cplx<T>* __restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* L = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int s0 = blockIdx.x * group;
+  const int ns = min(group, nst - s0);
+  const int64_t b = blockIdx.y;
+  const int len = ns * n3;
+  const cplx<T> zero = mk<T>(T(0), T(0));
+  for (int i = tid; i < len; i += kStickThreads) L[i] = zero;
+  __syncthreads();
+  const int d0 = off[s0], d1 = off[s0 + ns];
+  const int shift = s0 * n3;
+  const cplx<T>* src = psi + b * ldpsi;
+  for (int d = d0 + tid; d < d1; d += kStickThreads) L[gpos[d] - shift] = src[d];
+  __syncthreads();
+  lds_fft<T, kStickThreads, kStickPts>(L, ns, n3, 1, nullptr, pz, tw, +1, tid);
+  cplx<T>* w = work + b * band_stride + s0;
+  for (int i = tid; i < len; i += kStickThreads) {
+    const int z = i / ns;
+    const int l = i - z * ns;
+    w[(int64_t)z * nst + l] = L[l * n3 + z];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kStickThreads) void k_sticks_forward(
+    const cplx<T>* __restrict__ work, int64_t band_stride, const cplx<T>* __restrict__ psi,
+    int64_t ldpsi, const T* __restrict__ kinetic, cplx<T>* __restrict__ out,
+    int64_t ldout, int nst, int n3, const int* __restrict__ off,
+    const int* __restrict__ gpos, int group, LinePlan pz,
+    const cplx<T>* __restrict__ tw, T inv_n) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* L = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int s0 = blockIdx.x * group;
+  const int ns = min(group, nst - s0);
+  const int64_t b = blockIdx.y;
+  const int len = ns * n3;
+  const cplx<T>* w = work + b * band_stride + s0;
+  for (int i = tid; i < len; i += kStickThreads) {
+    const int z = i / ns;
+    const int l = i - z * ns;
+    L[l * n3 + z] = w[(int64_t)z * nst + l];
+  }
+  __syncthreads();
+  lds_fft<T, kStickThreads, kStickPts>(L, ns, n3, 1, nullptr, pz, tw, -1, tid);
+  const int d0 = off[s0], d1 = off[s0 + ns];
+  const int shift = s0 * n3;
+  const cplx<T>* src = psi + b * ldpsi;
+  cplx<T>* dst = out + b * ldout;
+  for (int d = d0 + tid; d < d1; d += kStickThreads) {
+    cplx<T> v = cscale(L[gpos[d] - shift], inv_n);
+    if (kinetic) {
+      const T k = kinetic[d];
+      const cplx<T> p = src[d];
+      v.x += k * p.x;
+      v.y += k * p.y;
+    }
+    dst[d] = v;
+  }
+}
+
+enum PlaneMode { kPlaneApply = 0, kPlaneReal = 1 };
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kPlaneThreads) void k_planes(
+    cplx<T>* __restrict__ work, int64_t band_stride, int nst,
+    const int* __restrict__ plane_of, const int* __restrict__ xcols, int nxcols,
+    const T* __restrict__ V, T* __restrict__ real_out, int n1, int n2,
+    LinePlan p1, LinePlan p2, const cplx<T>* __restrict__ tw1,
+    const cplx<T>* __restrict__ tw2) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* P = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int z = blockIdx.x;
+  const int64_t b = blockIdx.y;
+  const int np = n1 * n2;
+  const cplx<T> zero = mk<T>(T(0), T(0));
+  cplx<T>* w = work + b * band_stride + (int64_t)z * nst;
+  for (int i = tid; i < np; i += kPlaneThreads) P[i] = zero;
+  __syncthreads();
+  for (int s = tid; s < nst; s += kPlaneThreads) P[plane_of[s]] = w[s];
+  __syncthreads();
+  // backward along y (contiguous) for the active x columns
+  lds_fft<T, kPlaneThreads, kPlanePoints>(P, nxcols, n2, 1, xcols, p2, tw2, +1, tid);
+  // backward along x (stride n2) for every y row
+  lds_fft<T, kPlaneThreads, kPlanePoints>(P, n2, 1, n2, nullptr, p1, tw1, +1, tid);
+  if (MODE == kPlaneReal) {
+    T* o = real_out + (int64_t)z * np;
+    for (int i = tid; i < np; i += kPlaneThreads) o[i] = P[i].x;
+    return;
+  }
+  const T* v = V + (int64_t)z * np;
+  for (int i = tid; i < np; i += kPlaneThreads) P[i] = cscale(P[i], v[i]);
+  __syncthreads();
+  lds_fft<T, kPlaneThreads, kPlanePoints>(P, n2, 1, n2, nullptr, p1, tw1, -1, tid);
+  lds_fft<T, kPlaneThreads, kPlanePoints>(P, nxcols, n2, 1, xcols, p2, tw2, -1, tid);
+  for (int s = tid; s < nst; s += kPlaneThreads) w[s] = P[plane_of[s]];
+}
+
+template <typename T> const cplx<T>* twiddles(const Plan& p, int axis);
+template <> const double2* twiddles<double>(const Plan& p, int axis) { return p.d_tw64[axis]; }
+template <> const float2* twiddles<float>(const Plan& p, int axis) { return p.d_tw32[axis]; }
+
+template <typename T>
+static int check_fft_limits(const Plan& p, const StickTables& st) {
+  const int np = p.n1 * p.n2;
+  if (np * (int)sizeof(cplx<T>) > kMaxLdsBytes) return QEPW_ERR_UNSUPPORTED;
+  if (np > lds_fft_capacity(p.line[0], kPlaneThreads, kPlanePoints)) return QEPW_ERR_UNSUPPORTED;
+  if (np > lds_fft_capacity(p.line[1], kPlaneThreads, kPlanePoints)) return QEPW_ERR_UNSUPPORTED;
+  if (st.group * p.n3 > lds_fft_capacity(p.line[2], kStickThreads, kStickPts))
+    return QEPW_ERR_UNSUPPORTED;
+  return QEPW_OK;
+}
+
+int stick_group_for(int n3, int nst) {
+  int cap = (kStickThreads * 15) / (n3 > 0 ? n3 : 1);
+  int g = cap < 32 ? cap : 32;
+  if (g < 1) g = 1;
+  if (g > nst && nst > 0) g = nst;
+  return g;
+}
+
+template <typename T>
+static void set_lds_attrs(size_t stick_bytes, size_t plane_bytes) {
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sticks_inverse<T>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)stick_bytes);
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sticks_forward<T>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)stick_bytes);
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_planes<T, kPlaneApply>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_bytes);
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_planes<T, kPlaneReal>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_bytes);
+}
+
+size_t apply_local_band_bytes(const Plan& p, int precision) {
+  const size_t c = precision == QEPW_C128 ? sizeof(double2) : sizeof(float2);
+  return (size_t)p.n3 * (size_t)p.sphere.nst * c;
+}
+
+template <typename T>
+int apply_local_t(const Plan& p, const void* vectors, int64_t ldv, int nb,
+                  const void* potential, const void* kinetic, void* result,
+                  int64_t ldr, void* workspace, size_t ws_bytes, hipStream_t stream) {
+  const StickTables& st = p.sphere;
+  int rc = check_fft_limits<T>(p, st);
+  if (rc) return rc;
+  if (nb <= 0) return QEPW_OK;
+  const size_t band_bytes = (size_t)p.n3 * st.nst * sizeof(cplx<T>);
+  int chunk = (int)(ws_bytes / band_bytes);
+  if (chunk < 1) return QEPW_ERR_WORKSPACE;
+  if (chunk > 65535) chunk = 65535;
+  const size_t stick_lds = (size_t)st.group * p.n3 * sizeof(cplx<T>);
+  const size_t plane_lds = (size_t)p.n1 * p.n2 * sizeof(cplx<T>);
+  set_lds_attrs<T>(stick_lds, plane_lds);
+  const int64_t bstride = (int64_t)p.n3 * st.nst;
+  const T inv_n = T(1.0 / ((double)p.n1 * p.n2 * p.n3));
+  cplx<T>* work = static_cast<cplx<T>*>(workspace);
+  for (int b0 = 0; b0 < nb; b0 += chunk) {
+    const int nbc = (nb - b0) < chunk ? (nb - b0) : chunk;
+    const cplx<T>* src = static_cast<const cplx<T>*>(vectors) + (int64_t)b0 * ldv;
+    cplx<T>* dst = static_cast<cplx<T>*>(result) + (int64_t)b0 * ldr;
+    dim3 gs(st.ngroups, nbc), gp(p.n3, nbc);
+    hipLaunchKernelGGL(k_sticks_inverse<T>, gs, dim3(kStickThreads), stick_lds, stream,
+                       src, ldv, work, bstride, st.nst, p.n3, st.d_off, st.d_gpos,
+                       st.group, p.line[2], twiddles<T>(p, 2));
+    hipLaunchKernelGGL((k_planes<T, kPlaneApply>), gp, dim3(kPlaneThreads), plane_lds, stream,
+                       work, bstride, st.nst, st.d_plane, st.d_xcols, st.nxcols,
+                       static_cast<const T*>(potential), static_cast<T*>(nullptr), p.n1, p.n2,
+                       p.line[0], p.line[1], twiddles<T>(p, 0), twiddles<T>(p, 1));
+    hipLaunchKernelGGL(k_sticks_forward<T>, gs, dim3(kStickThreads), stick_lds, stream,
+                       work, bstride, src, ldv, static_cast<const T*>(kinetic), dst, ldr,
+                       st.nst, p.n3, st.d_off, st.d_gpos, st.group, p.line[2],
+                       twiddles<T>(p, 2), inv_n);
+  }
+  return hipGetLastError() == hipSuccess ? QEPW_OK : QEPW_ERR_HIP;
+}
+
+int apply_local(const Plan& p, int precision, const void* vectors, int64_t ldv, int nb,
+                const void* potential, const void* kinetic, void* result, int64_t ldr,
+                void* workspace, size_t ws_bytes, hipStream_t stream) {
+  if (precision == QEPW_C128)
+    return apply_local_t<double>(p, vectors, ldv, nb, potential, kinetic, result, ldr,
+                                 workspace, ws_bytes, stream);
+  if (precision == QEPW_C64)
+    return apply_local_t<float>(p, vectors, ldv, nb, potential, kinetic, result, ldr,
+                                workspace, ws_bytes, stream);
+  return QEPW_ERR_ARG;
+}
+
+size_t inverse_real_bytes(const Plan& p) {
+  return (size_t)p.n1 * p.n2 * p.n3 * sizeof(double2);
+}
+
+// V(r) = Re backward-FFT(V(G)) of a C-order (n1, n2, n3) complex grid, written
+// in the plane-major (n3, n1, n2) layout the plane kernel reads
+// (reference: _native_fft.pyx:2188-2202 inverse_real; basis.py:777-792).
+int inverse_real(const Plan& p, const void* grid_g, void* out_real, void* workspace,
+                 size_t ws_bytes, hipStream_t stream) {
+  const StickTables& st = p.full;
+  int rc = check_fft_limits<double>(p, st);
+  if (rc) return rc;
+  if (ws_bytes < inverse_real_bytes(p)) return QEPW_ERR_WORKSPACE;
+  const size_t stick_lds = (size_t)st.group * p.n3 * sizeof(double2);
+  const size_t plane_lds = (size_t)p.n1 * p.n2 * sizeof(double2);
+  set_lds_attrs<double>(stick_lds, plane_lds);
+  const int64_t bstride = (int64_t)p.n3 * st.nst;
+  double2* work = static_cast<double2*>(workspace);
+  hipLaunchKernelGGL(k_sticks_inverse<double>, dim3(st.ngroups, 1), dim3(kStickThreads),
+                     stick_lds, stream, static_cast<const double2*>(grid_g), (int64_t)0, work,
+                     bstride, st.nst, p.n3, st.d_off, st.d_gpos, st.group, p.line[2],
+                     twiddles<double>(p, 2));
+  hipLaunchKernelGGL((k_planes<double, kPlaneReal>), dim3(p.n3, 1), dim3(kPlaneThreads),
+                     plane_lds, stream, work, bstride, st.nst, st.d_plane, st.d_xcols,
+                     st.nxcols, static_cast<const double*>(nullptr),
+                     static_cast<double*>(out_real), p.n1, p.n2, p.line[0], p.line[1],
+                     twiddles<double>(p, 0), twiddles<double>(p, 1));
+  return hipGetLastError() == hipSuccess ? QEPW_OK : QEPW_ERR_HIP;
+}
+
+}  // namespace qepw

@@ -1,0 +1,246 @@
+// Complex GEMM on CDNA4 matrix cores: C = alpha op(A) B + beta C.
+//
+// Every complex product is four real MFMA products accumulated into separate
+// real / imaginary accumulators (no 3M trick, which loses accuracy):
+//   op N : re += Ar Br - Ai Bi ;  im += Ar Bi + Ai Br
+//   op C : re += Ar Br + Ai Bi ;  im += Ar Bi - Ai Br
+// using v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (FP32, FP32
+// accumulation).  A 256-thread workgroup owns a 64x64 output tile; each of
+// its four waves owns a 32x32 quarter (2x2 MFMA blocks of 16x16) and walks K
+// in steps of 16 staged through LDS.  Tall-skinny A^H B products (K = npw up
+// to ~24k, M,N <= 512) are split over K: every split writes its partial tile
+// to the workspace and a second kernel sums the splits in fixed order, so
+// results are bitwise reproducible run to run.
+//
+// MFMA 16x16x4 operand layout (lane l of a wave64): A element (row l%16,
+// k l/16); B element (k l/16, col l%16).  Accumulator v of lane l holds
+// column l%16 and row 4*(l/16)+v for the f32 instruction but row l/16+4v
+// for the f64 one (measured on gfx950 with an in-bounds probe).
+#include "qepw_internal.h"
+
+namespace qepw {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  using acc = v4d;
+  static __device__ __forceinline__ acc op(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <> struct Mfma<float> {
+  using acc = v4f;
+  static __device__ __forceinline__ acc op(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+constexpr int kTile = 64;
+constexpr int kBK = 16;
+constexpr int kGemmThreads = 256;
+constexpr int kPad = 1;
+
+template <typename T, bool CONJ_A>
+__global__ __launch_bounds__(kGemmThreads) void k_cgemm(
+    int64_t M, int64_t N, int64_t K, int64_t kchunk, const cplx<T>* __restrict__ A,
+    int64_t lda, const cplx<T>* __restrict__ B, int64_t ldb, cplx<T>* __restrict__ C,
+    int64_t ldc, cplx<T>* __restrict__ partial, T alpha_r, T alpha_i, T beta_r,
+    T beta_i) {
+  using Acc = typename Mfma<T>::acc;
+  __shared__ cplx<T> As[kBK][kTile + kPad];
+  __shared__ cplx<T> Bs[kBK][kTile + kPad];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave & 1) * 32;
+  const int wn = (wave >> 1) * 32;
+  const int64_t m0 = (int64_t)blockIdx.x * kTile;
+  const int64_t n0 = (int64_t)blockIdx.y * kTile;
+  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
+  const int64_t kend = (kbeg + kchunk) < K ? (kbeg + kchunk) : K;
+  const cplx<T> zero = mk<T>(T(0), T(0));
+
+  Acc re[2][2], im[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) { re[i][j][v] = T(0); im[i][j][v] = T(0); }
+
+  for (int64_t k0 = kbeg; k0 < kend; k0 += kBK) {
+#pragma unroll
+    for (int e = tid; e < kBK * kTile; e += kGemmThreads) {
+      int kk, mm;
+      if (CONJ_A) { kk = e & (kBK - 1); mm = e >> 4; }
+      else { mm = e & (kTile - 1); kk = e >> 6; }
+      const int64_t k = k0 + kk, m = m0 + mm;
+      cplx<T> v = zero;
+      if (k < kend && m < M) v = CONJ_A ? A[k + m * lda] : A[m + k * lda];
+      As[kk][mm] = v;
+      const int kb = e & (kBK - 1), nb = e >> 4;
+      const int64_t kq = k0 + kb, n = n0 + nb;
+      cplx<T> w = zero;
+      if (kq < kend && n < N) w = B[kq + n * ldb];
+      Bs[kb][nb] = w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kBK; ks += 4) {
+      const int kk = ks + (lane >> 4);
+      cplx<T> a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = As[kk][wm + 16 * i + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b[j] = Bs[kk][wn + 16 * j + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const T ai_re = CONJ_A ? a[i].y : -a[i].y;
+        const T ai_im = CONJ_A ? -a[i].y : a[i].y;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          re[i][j] = Mfma<T>::op(a[i].x, b[j].x, re[i][j]);
+          re[i][j] = Mfma<T>::op(ai_re, b[j].y, re[i][j]);
+          im[i][j] = Mfma<T>::op(a[i].x, b[j].y, im[i][j]);
+          im[i][j] = Mfma<T>::op(ai_im, b[j].x, im[i][j]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  const bool direct = gridDim.z == 1;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t row = m0 + wm + 16 * i + Mfma<T>::row(lane, v);
+        const int64_t col = n0 + wn + 16 * j + (lane & 15);
+        if (row >= M || col >= N) continue;
+        const T sr = re[i][j][v], si = im[i][j][v];
+        if (direct) {
+          cplx<T> out;
+          out.x = alpha_r * sr - alpha_i * si;
+          out.y = alpha_r * si + alpha_i * sr;
+          if (beta_r != T(0) || beta_i != T(0)) {
+            const cplx<T> c = C[row + col * ldc];
+            out.x += beta_r * c.x - beta_i * c.y;
+            out.y += beta_r * c.y + beta_i * c.x;
+          }
+          C[row + col * ldc] = out;
+        } else {
+          partial[((int64_t)blockIdx.z * N + col) * M + row] = mk<T>(sr, si);
+        }
+      }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_reduce_splits(
+    int64_t M, int64_t N, int splits, const cplx<T>* __restrict__ partial,
+    cplx<T>* __restrict__ C, int64_t ldc, T alpha_r, T alpha_i, T beta_r, T beta_i) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= M * N) return;
+  const int64_t col = idx / M, row = idx - col * M;
+  T sr = T(0), si = T(0);
+  for (int z = 0; z < splits; ++z) {
+    const cplx<T> p = partial[(int64_t)z * M * N + idx];
+    sr += p.x;
+    si += p.y;
+  }
+  cplx<T> out;
+  out.x = alpha_r * sr - alpha_i * si;
+  out.y = alpha_r * si + alpha_i * sr;
+  if (beta_r != T(0) || beta_i != T(0)) {
+    const cplx<T> c = C[row + col * ldc];
+    out.x += beta_r * c.x - beta_i * c.y;
+    out.y += beta_r * c.y + beta_i * c.x;
+  }
+  C[row + col * ldc] = out;
+}
+
+struct SplitPlan {
+  int64_t splits;
+  int64_t kchunk;
+};
+
+static SplitPlan split_plan(int op_a, int64_t m, int64_t n, int64_t k) {
+  const int64_t tiles = ((m + kTile - 1) / kTile) * ((n + kTile - 1) / kTile);
+  int64_t splits = 1;
+  if (op_a == QEPW_OP_C && k > 512) {
+    const int64_t want = (1024 + tiles - 1) / tiles;
+    const int64_t most = k / 256;
+    splits = want < most ? want : most;
+    if (splits < 1) splits = 1;
+  }
+  int64_t kchunk = (k + splits - 1) / splits;
+  kchunk = ((kchunk + kBK - 1) / kBK) * kBK;
+  if (kchunk < kBK) kchunk = kBK;
+  splits = (k + kchunk - 1) / kchunk;
+  if (splits < 1) splits = 1;
+  return {splits, kchunk};
+}
+
+size_t gemm_workspace_bytes(int precision, int op_a, int64_t m, int64_t n, int64_t k) {
+  const SplitPlan sp = split_plan(op_a, m, n, k);
+  if (sp.splits <= 1) return 0;
+  const size_t c = precision == QEPW_C128 ? sizeof(double2) : sizeof(float2);
+  return (size_t)sp.splits * (size_t)m * (size_t)n * c;
+}
+
+template <typename T>
+static int gemm_t(int op_a, int64_t m, int64_t n, int64_t k, const double* alpha,
+                  const void* a, int64_t lda, const void* b, int64_t ldb, const double* beta,
+                  void* c, int64_t ldc, void* ws, size_t ws_bytes, hipStream_t stream) {
+  const SplitPlan sp = split_plan(op_a, m, n, k);
+  if (sp.splits > 65535) return QEPW_ERR_UNSUPPORTED;
+  const size_t need = sp.splits > 1 ? (size_t)sp.splits * m * n * sizeof(cplx<T>) : 0;
+  if (ws_bytes < need) return QEPW_ERR_WORKSPACE;
+  const dim3 grid((unsigned)((m + kTile - 1) / kTile), (unsigned)((n + kTile - 1) / kTile),
+                  (unsigned)sp.splits);
+  const T ar = (T)alpha[0], ai = (T)alpha[1], br = (T)beta[0], bi = (T)beta[1];
+  cplx<T>* part = static_cast<cplx<T>*>(ws);
+  if (op_a == QEPW_OP_C)
+    hipLaunchKernelGGL((k_cgemm<T, true>), grid, dim3(kGemmThreads), 0, stream, m, n, k,
+                       sp.kchunk, static_cast<const cplx<T>*>(a), lda,
+                       static_cast<const cplx<T>*>(b), ldb, static_cast<cplx<T>*>(c), ldc,
+                       part, ar, ai, br, bi);
+  else
+    hipLaunchKernelGGL((k_cgemm<T, false>), grid, dim3(kGemmThreads), 0, stream, m, n, k,
+                       sp.kchunk, static_cast<const cplx<T>*>(a), lda,
+                       static_cast<const cplx<T>*>(b), ldb, static_cast<cplx<T>*>(c), ldc,
+                       part, ar, ai, br, bi);
+  if (sp.splits > 1) {
+    const int64_t total = m * n;
+    hipLaunchKernelGGL(k_reduce_splits<T>, dim3((unsigned)((total + 255) / 256)), dim3(256),
+                       0, stream, m, n, (int)sp.splits, part, static_cast<cplx<T>*>(c), ldc,
+                       ar, ai, br, bi);
+  }
+  return hipGetLastError() == hipSuccess ? QEPW_OK : QEPW_ERR_HIP;
+}
+
+int gemm(int precision, int op_a, int64_t m, int64_t n, int64_t k, const double* alpha,
+         const void* a, int64_t lda, const void* b, int64_t ldb, const double* beta,
+         void* c, int64_t ldc, void* ws, size_t ws_bytes, hipStream_t stream) {
+  if (m < 0 || n < 0 || k < 0) return QEPW_ERR_ARG;
+  if (op_a != QEPW_OP_N && op_a != QEPW_OP_C) return QEPW_ERR_ARG;
+  if (m == 0 || n == 0) return QEPW_OK;
+  if (ldc < m || ldb < (k > 0 ? k : 1)) return QEPW_ERR_ARG;
+  if (op_a == QEPW_OP_N ? lda < m : lda < (k > 0 ? k : 1)) return QEPW_ERR_ARG;
+  if ((m + kTile - 1) / kTile > 0x7fffffff || (n + kTile - 1) / kTile > 65535)
+    return QEPW_ERR_UNSUPPORTED;
+  if (precision == QEPW_C128)
+    return gemm_t<double>(op_a, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ws, ws_bytes,
+                          stream);
+  if (precision == QEPW_C64)
+    return gemm_t<float>(op_a, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ws, ws_bytes,
+                         stream);
+  return QEPW_ERR_ARG;
+}
+
+}  // namespace qepw

@@ -1,0 +1,50 @@
+// Internal declarations shared by the HIP translation units.
+#pragma once
+
+#include "qepw_common.h"
+#include "qepw_plan.h"
+#include "../../include/qepw.h"
+
+namespace qepw {
+
+constexpr int kMaxLdsBytes = 160 * 1024;
+
+// qepw_fft.hip
+int stick_group_for(int n3, int nst);
+size_t apply_local_band_bytes(const Plan& p, int precision);
+int apply_local(const Plan& p, int precision, const void* vectors, int64_t ldv, int nb,
+                const void* potential, const void* kinetic, void* result, int64_t ldr,
+                void* workspace, size_t ws_bytes, hipStream_t stream);
+size_t inverse_real_bytes(const Plan& p);
+int inverse_real(const Plan& p, const void* grid_g, void* out_real, void* workspace,
+                 size_t ws_bytes, hipStream_t stream);
+
+// qepw_gemm.hip
+size_t gemm_workspace_bytes(int precision, int op_a, int64_t m, int64_t n, int64_t k);
+int gemm(int precision, int op_a, int64_t m, int64_t n, int64_t k, const double* alpha,
+         const void* a, int64_t lda, const void* b, int64_t ldb, const double* beta,
+         void* c, int64_t ldc, void* workspace, size_t ws_bytes, hipStream_t stream);
+
+// qepw_vec.hip
+int column_squared_norms(int precision, const void* v, int64_t ldv, int64_t nrows, int nb,
+                         double* out, hipStream_t stream);
+int column_inner_products(int precision, const void* l, int64_t ldl, const void* r,
+                          int64_t ldr, int64_t nrows, int nb, double* out, hipStream_t stream);
+int subtract_band_energies(int precision, void* r, int64_t ldr, const void* x, int64_t ldx,
+                           const double* e, int64_t nrows, int nb, hipStream_t stream);
+int qe_precondition(int precision, const void* r, int64_t ldr, const int* cols,
+                    const double* e, const double* diag, void* out, int64_t ldo,
+                    int64_t nrows, int nb, hipStream_t stream);
+int normalize_selected_columns(int precision, const void* src, int64_t lds,
+                               const double* sq, const int* sel, void* dst, int64_t ldd,
+                               int64_t nrows, int nb, hipStream_t stream);
+int restart_applied(int precision, const void* x, int64_t ldx, const void* r, int64_t ldr,
+                    const double* e, void* hv, int64_t ldh, int64_t nrows, int nb,
+                    hipStream_t stream);
+int permute_rows(int src_precision, int dst_precision, int direction, const int* perm,
+                 const void* src, int64_t lds, void* dst, int64_t ldd, int64_t nrows, int nb,
+                 hipStream_t stream);
+int row_real_dot(const void* a, int64_t lda, const void* b, int64_t ldb, int64_t nrows,
+                 int ncols, double scale, double* acc, hipStream_t stream);
+
+}  // namespace qepw

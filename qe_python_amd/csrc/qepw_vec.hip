@@ -1,0 +1,288 @@
+// Bandwidth-bound Davidson kernels over (npw, nb) column-major blocks.
+//
+// Each reproduces one elementwise / reduction kernel of the reference's
+// native extension (qepy_pw/_native_fft.pyx); the formula is cited at each
+// kernel.  Reductions accumulate in FP64 for both storage precisions and use
+// a fixed per-column tree, so they are deterministic.
+#include "qepw_internal.h"
+
+namespace qepw {
+
+constexpr int kRedThreads = 256;
+
+template <typename T>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] += sh[tid + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// column_squared_norms (_native_fft.pyx:1153-1191): out[b] = sum_g |v_gb|^2
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_col_sqnorm(const cplx<T>* __restrict__ v,
+                                                            int64_t ldv, int64_t nrows,
+                                                            double* __restrict__ out) {
+  __shared__ double sh[kRedThreads];
+  const int64_t b = blockIdx.x;
+  const cplx<T>* col = v + b * ldv;
+  double acc = 0.0;
+  for (int64_t g = threadIdx.x; g < nrows; g += kRedThreads) {
+    const cplx<T> z = col[g];
+    acc += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
+  }
+  const double s = block_sum<T>(acc, sh);
+  if (threadIdx.x == 0) out[b] = s;
+}
+
+// column_inner_products (_native_fft.pyx:1241-1291): out[b] = <l_b | r_b>
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_col_inner(const cplx<T>* __restrict__ l,
+                                                           int64_t ldl,
+                                                           const cplx<T>* __restrict__ r,
+                                                           int64_t ldr, int64_t nrows,
+                                                           double* __restrict__ out) {
+  __shared__ double sh[kRedThreads];
+  const int64_t b = blockIdx.x;
+  double ar = 0.0, ai = 0.0;
+  for (int64_t g = threadIdx.x; g < nrows; g += kRedThreads) {
+    const cplx<T> x = l[b * ldl + g];
+    const cplx<T> y = r[b * ldr + g];
+    ar += (double)x.x * y.x + (double)x.y * y.y;
+    ai += (double)x.x * y.y - (double)x.y * y.x;
+  }
+  const double sr = block_sum<T>(ar, sh);
+  const double si = block_sum<T>(ai, sh);
+  if (threadIdx.x == 0) { out[2 * b] = sr; out[2 * b + 1] = si; }
+}
+
+// subtract_band_energies (_native_fft.pyx:1194-1238): r[:,b] -= e[b] x[:,b]
+template <typename T>
+__global__ void k_sub_energies(cplx<T>* __restrict__ r, int64_t ldr,
+                               const cplx<T>* __restrict__ x, int64_t ldx,
+                               const double* __restrict__ e, int64_t nrows) {
+  const int64_t b = blockIdx.y;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrows) return;
+  const T eb = (T)e[b];
+  const cplx<T> xv = x[b * ldx + g];
+  cplx<T> rv = r[b * ldr + g];
+  rv.x -= eb * xv.x;
+  rv.y -= eb * xv.y;
+  r[b * ldr + g] = rv;
+}
+
+// qe_precondition (_native_fft.pyx:852-899), QE g_psi in Rydberg units.
+template <typename T>
+__global__ void k_precondition(const cplx<T>* __restrict__ r, int64_t ldr,
+                               const int* __restrict__ cols, const double* __restrict__ e,
+                               const double* __restrict__ diag, cplx<T>* __restrict__ out,
+                               int64_t ldo, int64_t nrows) {
+  const int64_t j = blockIdx.y;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrows) return;
+  const int64_t c = cols ? cols[j] : j;
+  const double x = 2.0 * (diag[g] - e[j]);
+  const double d = 0.5 * (1.0 + x + sqrt(1.0 + (x - 1.0) * (x - 1.0)));
+  const double f = 2.0 / d;
+  const cplx<T> v = r[c * ldr + g];
+  out[j * ldo + g] = mk<T>((T)(f * (double)v.x), (T)(f * (double)v.y));
+}
+
+// normalize_selected_columns (_native_fft.pyx:1495-1530)
+template <typename T>
+__global__ void k_normalize_sel(const cplx<T>* __restrict__ src, int64_t lds,
+                                const double* __restrict__ sq, const int* __restrict__ sel,
+                                cplx<T>* __restrict__ dst, int64_t ldd, int64_t nrows) {
+  const int64_t j = blockIdx.y;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrows) return;
+  const int64_t c = sel[j];
+  const double inv = 1.0 / sqrt(sq[c]);
+  const cplx<T> v = src[c * lds + g];
+  dst[j * ldd + g] = mk<T>((T)((double)v.x * inv), (T)((double)v.y * inv));
+}
+
+// davidson restart (diagonalization.py:843-868): hv = x e + r
+template <typename T>
+__global__ void k_restart(const cplx<T>* __restrict__ x, int64_t ldx,
+                          const cplx<T>* __restrict__ r, int64_t ldr,
+                          const double* __restrict__ e, cplx<T>* __restrict__ hv, int64_t ldh,
+                          int64_t nrows) {
+  const int64_t b = blockIdx.y;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrows) return;
+  const T eb = (T)e[b];
+  const cplx<T> xv = x[b * ldx + g];
+  const cplx<T> rv = r[b * ldr + g];
+  hv[b * ldh + g] = mk<T>(xv.x * eb + rv.x, xv.y * eb + rv.y);
+}
+
+template <typename TS, typename TD>
+__global__ void k_permute(const int* __restrict__ perm, int direction,
+                          const cplx<TS>* __restrict__ src, int64_t lds,
+                          cplx<TD>* __restrict__ dst, int64_t ldd, int64_t nrows) {
+  const int64_t b = blockIdx.y;
+  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= nrows) return;
+  const int64_t p = perm ? perm[d] : d;
+  if (direction == 0) {
+    const cplx<TS> v = src[b * lds + p];
+    dst[b * ldd + d] = mk<TD>((TD)v.x, (TD)v.y);
+  } else {
+    const cplx<TS> v = src[b * lds + d];
+    dst[b * ldd + p] = mk<TD>((TD)v.x, (TD)v.y);
+  }
+}
+
+// accumulate_projector_diagonal (_native_fft.pyx:734-779)
+__global__ void k_row_real_dot(const double2* __restrict__ a, int64_t lda,
+                               const double2* __restrict__ b, int64_t ldb, int64_t nrows,
+                               int ncols, double scale, double* __restrict__ acc) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrows) return;
+  double s = 0.0;
+  for (int i = 0; i < ncols; ++i) {
+    const double2 x = a[i * lda + g];
+    const double2 y = b[i * ldb + g];
+    s += x.x * y.x + x.y * y.y;
+  }
+  acc[g] += scale * s;
+}
+
+static inline dim3 rows_grid(int64_t nrows, int nb) {
+  return dim3((unsigned)((nrows + 255) / 256), (unsigned)nb);
+}
+
+static inline int status() { return hipGetLastError() == hipSuccess ? QEPW_OK : QEPW_ERR_HIP; }
+
+#define QEPW_DISPATCH(prec, CALL_D, CALL_F)          \
+  do {                                               \
+    if ((prec) == QEPW_C128) { CALL_D; }             \
+    else if ((prec) == QEPW_C64) { CALL_F; }         \
+    else return QEPW_ERR_ARG;                        \
+  } while (0)
+
+int column_squared_norms(int precision, const void* v, int64_t ldv, int64_t nrows, int nb,
+                         double* out, hipStream_t stream) {
+  if (nb <= 0) return QEPW_OK;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_col_sqnorm<double>, dim3(nb), dim3(kRedThreads), 0, stream,
+                         static_cast<const double2*>(v), ldv, nrows, out),
+      hipLaunchKernelGGL(k_col_sqnorm<float>, dim3(nb), dim3(kRedThreads), 0, stream,
+                         static_cast<const float2*>(v), ldv, nrows, out));
+  return status();
+}
+
+int column_inner_products(int precision, const void* l, int64_t ldl, const void* r,
+                          int64_t ldr, int64_t nrows, int nb, double* out,
+                          hipStream_t stream) {
+  if (nb <= 0) return QEPW_OK;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_col_inner<double>, dim3(nb), dim3(kRedThreads), 0, stream,
+                         static_cast<const double2*>(l), ldl,
+                         static_cast<const double2*>(r), ldr, nrows, out),
+      hipLaunchKernelGGL(k_col_inner<float>, dim3(nb), dim3(kRedThreads), 0, stream,
+                         static_cast<const float2*>(l), ldl,
+                         static_cast<const float2*>(r), ldr, nrows, out));
+  return status();
+}
+
+int subtract_band_energies(int precision, void* r, int64_t ldr, const void* x, int64_t ldx,
+                           const double* e, int64_t nrows, int nb, hipStream_t stream) {
+  if (nb <= 0 || nrows <= 0) return QEPW_OK;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_sub_energies<double>, rows_grid(nrows, nb), dim3(256), 0, stream,
+                         static_cast<double2*>(r), ldr, static_cast<const double2*>(x), ldx,
+                         e, nrows),
+      hipLaunchKernelGGL(k_sub_energies<float>, rows_grid(nrows, nb), dim3(256), 0, stream,
+                         static_cast<float2*>(r), ldr, static_cast<const float2*>(x), ldx,
+                         e, nrows));
+  return status();
+}
+
+int qe_precondition(int precision, const void* r, int64_t ldr, const int* cols,
+                    const double* e, const double* diag, void* out, int64_t ldo,
+                    int64_t nrows, int nb, hipStream_t stream) {
+  if (nb <= 0 || nrows <= 0) return QEPW_OK;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_precondition<double>, rows_grid(nrows, nb), dim3(256), 0, stream,
+                         static_cast<const double2*>(r), ldr, cols, e, diag,
+                         static_cast<double2*>(out), ldo, nrows),
+      hipLaunchKernelGGL(k_precondition<float>, rows_grid(nrows, nb), dim3(256), 0, stream,
+                         static_cast<const float2*>(r), ldr, cols, e, diag,
+                         static_cast<float2*>(out), ldo, nrows));
+  return status();
+}
+
+int normalize_selected_columns(int precision, const void* src, int64_t lds,
+                               const double* sq, const int* sel, void* dst, int64_t ldd,
+                               int64_t nrows, int nb, hipStream_t stream) {
+  if (nb <= 0 || nrows <= 0) return QEPW_OK;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_normalize_sel<double>, rows_grid(nrows, nb), dim3(256), 0, stream,
+                         static_cast<const double2*>(src), lds, sq, sel,
+                         static_cast<double2*>(dst), ldd, nrows),
+      hipLaunchKernelGGL(k_normalize_sel<float>, rows_grid(nrows, nb), dim3(256), 0, stream,
+                         static_cast<const float2*>(src), lds, sq, sel,
+                         static_cast<float2*>(dst), ldd, nrows));
+  return status();
+}
+
+int restart_applied(int precision, const void* x, int64_t ldx, const void* r, int64_t ldr,
+                    const double* e, void* hv, int64_t ldh, int64_t nrows, int nb,
+                    hipStream_t stream) {
+  if (nb <= 0 || nrows <= 0) return QEPW_OK;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_restart<double>, rows_grid(nrows, nb), dim3(256), 0, stream,
+                         static_cast<const double2*>(x), ldx,
+                         static_cast<const double2*>(r), ldr, e,
+                         static_cast<double2*>(hv), ldh, nrows),
+      hipLaunchKernelGGL(k_restart<float>, rows_grid(nrows, nb), dim3(256), 0, stream,
+                         static_cast<const float2*>(x), ldx,
+                         static_cast<const float2*>(r), ldr, e,
+                         static_cast<float2*>(hv), ldh, nrows));
+  return status();
+}
+
+int permute_rows(int sp, int dp, int direction, const int* perm, const void* src,
+                 int64_t lds, void* dst, int64_t ldd, int64_t nrows, int nb,
+                 hipStream_t stream) {
+  if (nb <= 0 || nrows <= 0) return QEPW_OK;
+  if (direction != 0 && direction != 1) return QEPW_ERR_ARG;
+  const dim3 g = rows_grid(nrows, nb);
+  if (sp == QEPW_C128 && dp == QEPW_C128)
+    hipLaunchKernelGGL((k_permute<double, double>), g, dim3(256), 0, stream, perm, direction,
+                       static_cast<const double2*>(src), lds, static_cast<double2*>(dst), ldd,
+                       nrows);
+  else if (sp == QEPW_C128 && dp == QEPW_C64)
+    hipLaunchKernelGGL((k_permute<double, float>), g, dim3(256), 0, stream, perm, direction,
+                       static_cast<const double2*>(src), lds, static_cast<float2*>(dst), ldd,
+                       nrows);
+  else if (sp == QEPW_C64 && dp == QEPW_C128)
+    hipLaunchKernelGGL((k_permute<float, double>), g, dim3(256), 0, stream, perm, direction,
+                       static_cast<const float2*>(src), lds, static_cast<double2*>(dst), ldd,
+                       nrows);
+  else if (sp == QEPW_C64 && dp == QEPW_C64)
+    hipLaunchKernelGGL((k_permute<float, float>), g, dim3(256), 0, stream, perm, direction,
+                       static_cast<const float2*>(src), lds, static_cast<float2*>(dst), ldd,
+                       nrows);
+  else
+    return QEPW_ERR_ARG;
+  return status();
+}
+
+int row_real_dot(const void* a, int64_t lda, const void* b, int64_t ldb, int64_t nrows,
+                 int ncols, double scale, double* acc, hipStream_t stream) {
+  if (nrows <= 0) return QEPW_OK;
+  hipLaunchKernelGGL(k_row_real_dot, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0,
+                     stream, static_cast<const double2*>(a), lda,
+                     static_cast<const double2*>(b), ldb, nrows, ncols, scale, acc);
+  return status();
+}
+
+}  // namespace qepw

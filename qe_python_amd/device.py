@@ -1,0 +1,233 @@
+"""Device buffers and thin wrappers over the C ABI.
+
+PyTorch-ROCm is used only as the allocator and stream provider: device
+matrices are torch tensors whose raw pointers go to ``libqepw_hip.so``; no
+torch arithmetic runs on the hot path.
+
+Matrix convention: a column-major (nrows x ncols) block with leading
+dimension ld is a torch tensor of shape (ncols, ld) with unit stride along
+dim 1, so ``t[a:b]`` is the column block a:b (as in the reference's
+F-ordered ``basis_storage[:, a:b]``, ``qepy_pw/diagonalization.py:632-637``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+C128 = torch.complex128
+C64 = torch.complex64
+
+
+def precision_code(dtype: torch.dtype) -> int:
+    if dtype == C128:
+        return nat.C128
+    if dtype == C64:
+        return nat.C64
+    raise ValueError(f"unsupported complex dtype {dtype}")
+
+
+def real_dtype(dtype: torch.dtype) -> torch.dtype:
+    return torch.float64 if dtype == C128 else torch.float32
+
+
+def stream_handle(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_device(device=None) -> torch.device:
+    """Resolve the HIP device; fail loudly when there is none."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device is visible: the qe_python_amd device path "
+                           "has no CPU fallback")
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    nat.load()
+    return dev
+
+
+def ld_of(t: torch.Tensor) -> int:
+    if t.dim() != 2 or (t.shape[0] > 1 and t.shape[1] > 1 and t.stride(1) != 1):
+        raise ValueError("device matrix must be a (ncols, ld) tensor with unit inner stride")
+    if t.shape[0] > 1:
+        return max(1, int(t.stride(0)))
+    return max(1, int(t.shape[1]), int(t.stride(0)))
+
+
+class Workspace:
+    """Grow-only device scratch (the device analogue of ``FFTScratchPool``,
+    ``qepy_pw/basis.py:407-454``)."""
+
+    def __init__(self, device: torch.device):
+        self.device = device
+        self.buffer = torch.empty(0, dtype=torch.uint8, device=device)
+
+    def get(self, nbytes: int) -> torch.Tensor:
+        nbytes = max(int(nbytes), 256)
+        if self.buffer.numel() < nbytes:
+            self.buffer = torch.empty(int(nbytes * 1.25) + 4096, dtype=torch.uint8,
+                                      device=self.device)
+        return self.buffer
+
+
+def gemm(op_a: int, m: int, n: int, k: int, alpha, a: torch.Tensor, b: torch.Tensor,
+         beta, c: torch.Tensor, ws: Workspace) -> None:
+    """c = alpha op(a) b + beta c on the device (column-major views)."""
+    lib = nat.load()
+    prec = precision_code(c.dtype)
+    if a.dtype != c.dtype or b.dtype != c.dtype:
+        raise ValueError("gemm operands must share one complex dtype")
+    need = lib.qepw_gemm_workspace(prec, op_a, m, n, k)
+    buf = ws.get(need)
+    rc = lib.qepw_gemm(prec, op_a, m, n, k, nat.complex_pair(alpha), a.data_ptr(), ld_of(a),
+                       b.data_ptr(), ld_of(b), nat.complex_pair(beta), c.data_ptr(), ld_of(c),
+                       buf.data_ptr(), buf.numel(), stream_handle(c.device))
+    nat.check(rc, "qepw_gemm")
+
+
+def column_squared_norms(v: torch.Tensor, nrows: int, out: torch.Tensor | None = None) -> torch.Tensor:
+    nb = v.shape[0]
+    if out is None:
+        out = torch.empty(nb, dtype=torch.float64, device=v.device)
+    rc = nat.load().qepw_column_squared_norms(precision_code(v.dtype), v.data_ptr(), ld_of(v),
+                                              nrows, nb, out.data_ptr(),
+                                              stream_handle(v.device))
+    nat.check(rc, "qepw_column_squared_norms")
+    return out
+
+
+def column_inner_products(l: torch.Tensor, r: torch.Tensor, nrows: int) -> torch.Tensor:
+    nb = l.shape[0]
+    out = torch.empty(2 * nb, dtype=torch.float64, device=l.device)
+    rc = nat.load().qepw_column_inner_products(precision_code(l.dtype), l.data_ptr(), ld_of(l),
+                                               r.data_ptr(), ld_of(r), nrows, nb,
+                                               out.data_ptr(), stream_handle(l.device))
+    nat.check(rc, "qepw_column_inner_products")
+    return out
+
+
+def subtract_band_energies(r: torch.Tensor, x: torch.Tensor, e: torch.Tensor, nrows: int) -> None:
+    rc = nat.load().qepw_subtract_band_energies(precision_code(r.dtype), r.data_ptr(), ld_of(r),
+                                                x.data_ptr(), ld_of(x), e.data_ptr(), nrows,
+                                                r.shape[0], stream_handle(r.device))
+    nat.check(rc, "qepw_subtract_band_energies")
+
+
+def qe_precondition(r: torch.Tensor, cols: torch.Tensor | None, e: torch.Tensor,
+                    diag: torch.Tensor, out: torch.Tensor, nrows: int) -> None:
+    rc = nat.load().qepw_qe_precondition(precision_code(r.dtype), r.data_ptr(), ld_of(r),
+                                         None if cols is None else cols.data_ptr(),
+                                         e.data_ptr(), diag.data_ptr(), out.data_ptr(),
+                                         ld_of(out), nrows, out.shape[0],
+                                         stream_handle(r.device))
+    nat.check(rc, "qepw_qe_precondition")
+
+
+def normalize_selected_columns(src: torch.Tensor, sq: torch.Tensor, sel: torch.Tensor,
+                               dst: torch.Tensor, nrows: int) -> None:
+    rc = nat.load().qepw_normalize_selected_columns(precision_code(src.dtype), src.data_ptr(),
+                                                    ld_of(src), sq.data_ptr(), sel.data_ptr(),
+                                                    dst.data_ptr(), ld_of(dst), nrows,
+                                                    dst.shape[0], stream_handle(src.device))
+    nat.check(rc, "qepw_normalize_selected_columns")
+
+
+def restart_applied(x: torch.Tensor, r: torch.Tensor, e: torch.Tensor, hv: torch.Tensor,
+                    nrows: int) -> None:
+    rc = nat.load().qepw_restart_applied(precision_code(x.dtype), x.data_ptr(), ld_of(x),
+                                         r.data_ptr(), ld_of(r), e.data_ptr(), hv.data_ptr(),
+                                         ld_of(hv), nrows, x.shape[0], stream_handle(x.device))
+    nat.check(rc, "qepw_restart_applied")
+
+
+def permute_rows(direction: int, perm: torch.Tensor | None, src: torch.Tensor,
+                 dst: torch.Tensor, nrows: int) -> None:
+    """direction 0: dst[d] = src[perm[d]]; 1: dst[perm[d]] = src[d]."""
+    rc = nat.load().qepw_permute_rows(precision_code(src.dtype), precision_code(dst.dtype),
+                                      direction, None if perm is None else perm.data_ptr(),
+                                      src.data_ptr(), ld_of(src), dst.data_ptr(), ld_of(dst),
+                                      nrows, src.shape[0], stream_handle(src.device))
+    nat.check(rc, "qepw_permute_rows")
+
+
+def accumulate_projector_diagonal(coupled: torch.Tensor, beta: torch.Tensor, nrows: int,
+                                  multiplicity: float, diagonal: torch.Tensor) -> None:
+    rc = nat.load().qepw_accumulate_projector_diagonal(
+        coupled.data_ptr(), ld_of(coupled), beta.data_ptr(), ld_of(beta), nrows,
+        coupled.shape[0], float(multiplicity), diagonal.data_ptr(),
+        stream_handle(coupled.device))
+    nat.check(rc, "qepw_accumulate_projector_diagonal")
+
+
+class DeviceFFTPlan:
+    """Device plan of the sphere-pruned V_loc FFT for one k-point basis
+    (device counterpart of the serial ``LocalPotentialWorkspace`` tables,
+    ``qepy_pw/basis.py:460-555``)."""
+
+    def __init__(self, miller: np.ndarray, shape, device: torch.device):
+        lib = nat.load()
+        self.device = device
+        miller = np.ascontiguousarray(miller, dtype=np.int32)
+        if miller.ndim != 2 or miller.shape[1] != 3:
+            raise ValueError("indices must have shape (nplane, 3)")
+        self.shape = tuple(int(s) for s in shape)
+        self.npw = int(miller.shape[0])
+        handle = ctypes.c_void_p()
+        rc = lib.qepw_plan_create(miller.ctypes.data, self.npw, *self.shape,
+                                  device.index, ctypes.byref(handle))
+        nat.check(rc, "qepw_plan_create")
+        self.handle = handle
+        info = (ctypes.c_int64 * 8)()
+        nat.check(lib.qepw_plan_query(handle, info), "qepw_plan_query")
+        self.sticks = int(info[4])
+        self.active_x = int(info[5])
+        self.stick_group = int(info[6])
+        perm = np.empty(self.npw, dtype=np.int32)
+        nat.check(lib.qepw_plan_permutation(handle, perm.ctypes.data), "qepw_plan_permutation")
+        self.perm = perm
+        self.perm_device = torch.from_numpy(perm).to(device)
+
+    def __del__(self):
+        handle = getattr(self, "handle", None)
+        if handle is not None and nat._LIB is not None:
+            nat._LIB.qepw_plan_destroy(handle)
+            self.handle = None
+
+    def workspace_bytes(self, dtype: torch.dtype, nb: int) -> int:
+        return int(nat.load().qepw_apply_local_workspace(self.handle, precision_code(dtype), nb))
+
+    def apply_local(self, vectors: torch.Tensor, result: torch.Tensor, potential: torch.Tensor,
+                    kinetic: torch.Tensor | None, ws: Workspace, max_bands: int = 256) -> None:
+        nb = vectors.shape[0]
+        if nb == 0:
+            return
+        prec = precision_code(vectors.dtype)
+        if result.dtype != vectors.dtype or potential.dtype != real_dtype(vectors.dtype):
+            raise ValueError("local-potential operands disagree in precision")
+        need = self.workspace_bytes(vectors.dtype, min(nb, max_bands))
+        buf = ws.get(need)
+        rc = nat.load().qepw_apply_local(
+            self.handle, prec, vectors.data_ptr(), ld_of(vectors), nb, potential.data_ptr(),
+            None if kinetic is None else kinetic.data_ptr(), result.data_ptr(), ld_of(result),
+            buf.data_ptr(), min(buf.numel(), need), stream_handle(vectors.device))
+        nat.check(rc, "qepw_apply_local")
+
+    def inverse_real(self, potential_g: torch.Tensor, ws: Workspace) -> torch.Tensor:
+        """(n3, n1, n2) float64 = Re backward FFT of a C-order complex grid."""
+        if potential_g.dtype != C128 or tuple(potential_g.shape) != self.shape:
+            raise ValueError("potential_g must be a complex128 grid of the plan's shape")
+        g = potential_g.contiguous()
+        n1, n2, n3 = self.shape
+        out = torch.empty((n3, n1, n2), dtype=torch.float64, device=self.device)
+        lib = nat.load()
+        need = lib.qepw_inverse_real_workspace(self.handle)
+        buf = ws.get(need)
+        rc = lib.qepw_inverse_real(self.handle, g.data_ptr(), out.data_ptr(), buf.data_ptr(),
+                                   buf.numel(), stream_handle(self.device))
+        nat.check(rc, "qepw_inverse_real")
+        return out

@@ -1,0 +1,700 @@
+"""Drop-in, device-resident replacement of ``qepy_pw/diagonalization.py``.
+
+Same names, signatures, argument meaning and exception types as the
+reference module for the h_psi / Davidson path, so that
+``qepy_pw/pw/scf.py:29-40`` can bind them unchanged:
+
+* ``PlaneWaveHamiltonian``  (reference ``diagonalization.py:124-369``)
+* ``davidson`` / ``DavidsonResult`` (``diagonalization.py:527-880, 113-121``)
+* ``FactorizedProjectorTerm`` / ``PackedProjectorTerm`` / ``ProjectorTerm``
+  (``diagonalization.py:21-43``)
+* ``_lowest_generalized_eigh`` / ``_matmul_f`` (``diagonalization.py:74-110``)
+
+Design (one GPU):
+
+* The Hamiltonian owns a device FFT plan whose rows are in "device order"
+  (plane waves grouped by z-stick), V(r) in plane-major (n3, n1, n2) layout,
+  the kinetic diagonal, and all projector terms packed into one device beta
+  (npw, P) plus a block-diagonal D.  ``apply``/``apply_into`` accept host
+  arrays in the reference's row order (or device tensors) and permute at the
+  boundary.
+* ``davidson`` keeps ``basis_storage``/``applied_storage`` on the device in
+  device order.  h_psi, every subspace product (MFMA GEMMs), the residual,
+  preconditioner, norms and normalisation run as HIP kernels.  Only the
+  reduced (nact x nact) H/S rows come to the host, where the FP64
+  Cholesky + Hermitian eigensolve (LAPACK zhegvd, the same routine the
+  reference calls from ``_native_fft.pyx:667-691``) runs, and the Ritz
+  coefficients go back.  Convergence flags are host scalars, as in the
+  reference.
+* Operators that are not a device ``PlaneWaveHamiltonian`` are still
+  accepted: their blocks are copied to the host for each application.
+
+There is no CPU fallback: without a HIP device or without
+``libqepw_hip.so`` every entry point raises.
+"""
+
+from __future__ import annotations
+
+from contextlib import nullcontext
+from dataclasses import dataclass
+from typing import Callable
+
+import numpy as np
+import scipy.linalg as sla
+import torch
+
+from . import _native as nat
+from . import device as dv
+
+BlockOperator = Callable[[np.ndarray], np.ndarray]
+BlockOperatorInto = Callable[[np.ndarray, np.ndarray], np.ndarray]
+MaterializedProjectorTerm = tuple[np.ndarray, np.ndarray]
+
+
+@dataclass(frozen=True)
+class FactorizedProjectorTerm:
+    """Species-centered beta functions plus atom-dependent phase columns."""
+
+    beta: np.ndarray
+    phases: np.ndarray
+    coupling: np.ndarray
+
+
+@dataclass(frozen=True)
+class PackedProjectorTerm:
+    """QE-style atom-packed beta matrix with a precomputed diagonal."""
+
+    beta: np.ndarray
+    coupling: np.ndarray
+    diagonal: np.ndarray
+
+
+ProjectorTerm = MaterializedProjectorTerm | FactorizedProjectorTerm | PackedProjectorTerm
+
+
+@dataclass(frozen=True)
+class DavidsonResult:
+    eigenvalues: np.ndarray
+    eigenvectors: np.ndarray
+    residual_norms: np.ndarray
+    iterations: float
+    converged: bool
+    number_unconverged: int = 0
+    hamiltonian_applications: int = 0
+
+
+class _SerialContext:
+    """The reference's serial ``MPIContext`` (``qepy_pw/mpi.py:14-31,129-136``)."""
+
+    rank = 0
+    size = 1
+    comm = None
+
+    def sum_array(self, value):
+        return np.ascontiguousarray(value)
+
+
+_SERIAL = _SerialContext()
+
+
+def _matmul_f(left: np.ndarray, right: np.ndarray) -> np.ndarray:
+    """Host product into a column-major block (``diagonalization.py:74-80``)."""
+    result = np.empty((left.shape[0], right.shape[1]), dtype=np.complex128, order="F")
+    np.matmul(left, right, out=result)
+    return result
+
+
+def _lowest_generalized_eigh(hamiltonian, overlap, roots):
+    """Lowest ``roots`` pairs of H c = e S c by LAPACK zhegvd(itype 1, 'V', 'L')
+    on the host (``_native_fft.pyx:667-691``; ``diagonalization.py:83-110``).
+    A non-positive-definite overlap raises ``numpy.linalg.LinAlgError``."""
+    h = np.array(hamiltonian, dtype=np.complex128, order="F", copy=True)
+    s = np.array(overlap, dtype=np.complex128, order="F", copy=True)
+    try:
+        values, vectors = sla.eigh(h, s, lower=True, type=1, driver="gvd",
+                                   overwrite_a=True, overwrite_b=True, check_finite=False)
+    except sla.LinAlgError as exc:  # scipy's LinAlgError subclasses numpy's
+        raise np.linalg.LinAlgError(f"zhegvd failed: {exc}") from exc
+    return values[:roots].copy(), np.asfortranarray(vectors[:, :roots])
+
+
+def _measure(timers, label, device=None):
+    if timers is None:
+        return nullcontext()
+    return _SyncedMeasure(timers, label, device)
+
+
+class _SyncedMeasure:
+    """Reference timer label around device work, synchronised on exit so the
+    wall time covers the kernels (``qepy_pw/timing.py:16-35``)."""
+
+    def __init__(self, timers, label, device):
+        self.inner = timers.measure(label)
+        self.device = device
+
+    def __enter__(self):
+        return self.inner.__enter__()
+
+    def __exit__(self, *exc):
+        if self.device is not None:
+            torch.cuda.synchronize(self.device)
+        return self.inner.__exit__(*exc)
+
+
+class _PrecisionResources:
+    """Device operands of H at one precision."""
+
+    def __init__(self, potential, kinetic, beta, coupling):
+        self.potential = potential
+        self.kinetic = kinetic
+        self.beta = beta
+        self.coupling = coupling
+        self.nproj = 0 if beta is None else int(beta.shape[0])
+
+
+class PlaneWaveHamiltonian:
+    """Apply scalar norm-conserving H on the GPU without an Npw-square array.
+
+    Arguments follow ``qepy_pw/diagonalization.py:127-137``.  Keyword-only
+    extras: ``device`` (HIP device, default the current one) and
+    ``precision`` ("fp64" or "fp32") of ``apply``/``apply_into``.
+    """
+
+    def __init__(self, basis, potential_g, projector_terms=(), local_workspace=None,
+                 timers=None, real_potential=None, native_potential_layout=False,
+                 potential_average=None, *, device=None, precision: str = "fp64"):
+        self.device = dv.require_device(device)
+        self.basis = basis
+        self.potential_g = np.asarray(potential_g)
+        self.local_workspace = local_workspace
+        self.timers = timers
+        mpi = getattr(local_workspace, "mpi", None)
+        self.mpi = mpi if mpi is not None else _SERIAL
+        if getattr(self.mpi, "size", 1) > 1:
+            raise ValueError("the device Hamiltonian runs on one GPU; a distributed "
+                             "LocalPotentialWorkspace is not supported")
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.precision = precision
+        shape = (tuple(local_workspace.shape) if local_workspace is not None
+                 else tuple(self.potential_g.shape))
+        if len(shape) != 3:
+            raise ValueError("the FFT grid shape must be three-dimensional")
+        self.shape = shape
+        indices = np.asarray(basis.indices)
+        self.npw = int(len(indices))
+        self.local_rows = np.arange(self.npw, dtype=np.int32)
+        self.local_kinetic = np.asarray(basis.kinetic, dtype=np.float64)
+        self.native_potential_layout = bool(native_potential_layout)
+        self.potential_average = (float(potential_average) if potential_average is not None
+                                  else float(np.real(self.potential_g.flat[0])))
+        self.projector_terms = tuple(projector_terms)
+        self.plan = dv.DeviceFFTPlan(indices, shape, self.device)
+        self.workspace = dv.Workspace(self.device)
+        perm = self.plan.perm
+        if real_potential is not None:
+            rp = np.asarray(real_potential, dtype=np.float64)
+            if self.native_potential_layout:
+                plane_major = np.ascontiguousarray(rp.reshape(shape[2], shape[0], shape[1]))
+            else:
+                plane_major = np.ascontiguousarray(np.transpose(rp.reshape(shape), (2, 0, 1)))
+            self._potential64 = torch.from_numpy(plane_major).to(self.device)
+            self._real_potential_host = rp
+        else:
+            vg = torch.from_numpy(np.ascontiguousarray(self.potential_g, dtype=np.complex128))
+            self._potential64 = self.plan.inverse_real(vg.to(self.device), self.workspace)
+            self._real_potential_host = None
+        kin = torch.from_numpy(np.ascontiguousarray(self.local_kinetic[perm])).to(self.device)
+        beta, coupling = self._pack_projectors(perm)
+        self._res = {torch.complex128: _PrecisionResources(self._potential64, kin, beta, coupling)}
+        self.applications = 0
+
+    # -- set-up -----------------------------------------------------------
+    def _pack_projectors(self, perm):
+        """All terms as one device beta (P, npw) [column-major npw x P, device
+        order] and a block-diagonal D (P, P), the packed form of
+        ``pw/scf.py:633-686`` for every term type."""
+        blocks, couplings = [], []
+        for term in self.projector_terms:
+            if isinstance(term, PackedProjectorTerm) or (
+                    hasattr(term, "diagonal") and hasattr(term, "beta")):
+                blocks.append(np.asarray(term.beta))
+                couplings.append(np.asarray(term.coupling))
+            elif isinstance(term, FactorizedProjectorTerm) or hasattr(term, "phases"):
+                beta = np.asarray(term.beta)
+                for atom in range(term.phases.shape[1]):
+                    blocks.append(beta * np.asarray(term.phases)[:, atom, None])
+                    couplings.append(np.asarray(term.coupling))
+            else:
+                beta, coupling = term
+                blocks.append(np.asarray(beta))
+                couplings.append(np.asarray(coupling))
+        blocks = [b for b in blocks if b.shape[1]]
+        couplings = [c for c in couplings if c.shape[0]]
+        if not blocks:
+            return None, None
+        for b in blocks:
+            if b.shape[0] != self.npw:
+                raise ValueError("projector rows must match the plane-wave basis")
+        nproj = sum(b.shape[1] for b in blocks)
+        beta_host = np.empty((nproj, self.npw), dtype=np.complex128)
+        d_host = np.zeros((nproj, nproj), dtype=np.complex128)
+        off = 0
+        for b, c in zip(blocks, couplings):
+            k = b.shape[1]
+            beta_host[off:off + k] = b[perm].T
+            d_host[off:off + k, off:off + k] = c
+            off += k
+        beta = torch.from_numpy(beta_host).to(self.device)
+        # column-major D: torch row j holds column j of D
+        coupling = torch.from_numpy(np.ascontiguousarray(d_host.T)).to(self.device)
+        return beta, coupling
+
+    def _resources(self, dtype: torch.dtype) -> _PrecisionResources:
+        res = self._res.get(dtype)
+        if res is None:
+            base = self._res[torch.complex128]
+            res = _PrecisionResources(
+                base.potential.to(torch.float32), base.kinetic.to(torch.float32),
+                None if base.beta is None else base.beta.to(torch.complex64),
+                None if base.coupling is None else base.coupling.to(torch.complex64))
+            self._res[dtype] = res
+        return res
+
+    # -- reference attributes ----------------------------------------------
+    @property
+    def real_potential(self) -> np.ndarray:
+        if self._real_potential_host is None:
+            n1, n2, n3 = self.shape
+            plane = self._potential64.cpu().numpy()
+            self._real_potential_host = np.ascontiguousarray(np.transpose(plane, (1, 2, 0)))
+        return self._real_potential_host
+
+    @property
+    def diagonal(self) -> np.ndarray:
+        """Hamiltonian diagonal used by QE's ``usnldiag`` preconditioner
+        (``diagonalization.py:205-229``); non-packed terms are accumulated on
+        the device (``accumulate_projector_diagonal``)."""
+        result = self.local_kinetic + self.potential_average
+        acc = None
+        for term in self.projector_terms:
+            if isinstance(term, PackedProjectorTerm):
+                result = result + term.diagonal
+                continue
+            if isinstance(term, FactorizedProjectorTerm):
+                beta, coupling, mult = term.beta, term.coupling, term.phases.shape[1]
+            else:
+                (beta, coupling), mult = term, 1
+            beta = np.asarray(beta)
+            if beta.shape[1] == 0:
+                continue
+            if acc is None:
+                acc = torch.zeros(self.npw, dtype=torch.float64, device=self.device)
+            b = torch.from_numpy(np.ascontiguousarray(beta.T, dtype=np.complex128)).to(self.device)
+            d = torch.from_numpy(np.ascontiguousarray(np.asarray(coupling, np.complex128).T)).to(self.device)
+            coupled = torch.empty_like(b)
+            dv.gemm(nat.OP_N, self.npw, b.shape[0], b.shape[0], 1.0, b, d, 0.0, coupled,
+                    self.workspace)
+            dv.accumulate_projector_diagonal(coupled, b, self.npw, mult, acc)
+        if acc is not None:
+            result = result + acc.cpu().numpy()
+        return result
+
+    # -- device-order interface ---------------------------------------------
+    def device_apply(self, src: torch.Tensor, dst: torch.Tensor) -> None:
+        """dst = H src for (nb, npw) device-order tensors (complex128 or
+        complex64); ``dst`` must not alias ``src``."""
+        nb = int(src.shape[0])
+        if nb == 0:
+            return
+        res = self._resources(src.dtype)
+        with _measure(self.timers, "vloc_psi", self.device):
+            self.plan.apply_local(src, dst, res.potential, res.kinetic, self.workspace)
+        if res.beta is not None:
+            p = res.nproj
+            with _measure(self.timers, "h_psi:calbec", self.device):
+                becp = torch.empty((nb, p), dtype=src.dtype, device=self.device)
+                dv.gemm(nat.OP_C, p, nb, self.npw, 1.0, res.beta, src, 0.0, becp, self.workspace)
+            with _measure(self.timers, "add_vuspsi", self.device):
+                coupled = torch.empty_like(becp)
+                dv.gemm(nat.OP_N, p, nb, p, 1.0, res.coupling, becp, 0.0, coupled, self.workspace)
+                dv.gemm(nat.OP_N, self.npw, nb, p, 1.0, res.beta, coupled, 1.0, dst,
+                        self.workspace)
+        self.applications += nb
+
+    def to_device(self, vectors, dtype=torch.complex128) -> torch.Tensor:
+        """(npw, nb) reference-order host array or device tensor -> (nb, npw)
+        device-order tensor."""
+        return _upload_block(vectors, self.plan.perm_device, self.npw, self.device, dtype)
+
+    def from_device(self, block: torch.Tensor) -> np.ndarray:
+        """(nb, npw) device-order tensor -> (npw, nb) F-order host array."""
+        return _download_block(block, self.plan.perm_device, self.npw)
+
+    # -- reference interface -------------------------------------------------
+    def apply(self, coefficients):
+        with _measure(self.timers, "h_psi"):
+            return self._apply(coefficients)
+
+    def apply_into(self, coefficients, out):
+        """Apply H directly into caller-owned storage (``diagonalization.py:237-244``)."""
+        with _measure(self.timers, "h_psi"):
+            return self._apply(coefficients, out=out)
+
+    def _apply(self, coefficients, out=None):
+        dtype = torch.complex128 if self.precision == "fp64" else torch.complex64
+        if isinstance(coefficients, torch.Tensor):
+            vec = coefficients
+            was_vector = vec.dim() == 1
+            if was_vector:
+                vec = vec[:, None]
+            if vec.dim() != 2 or vec.shape[0] != self.npw:
+                raise ValueError("coefficients must have shape (nplane,) or (nplane, nvec)")
+            src = self.to_device(vec, dtype)
+            dst = torch.empty_like(src)
+            self.device_apply(src, dst)
+            res = torch.empty((dst.shape[0], self.npw), dtype=torch.complex128, device=self.device)
+            dv.permute_rows(1, self.plan.perm_device, dst, res, self.npw)
+            res = res.T
+            if out is not None:
+                target = out[:, None] if (was_vector and out.dim() == 1) else out
+                target.copy_(res)
+                return out
+            return res[:, 0] if was_vector else res
+        vectors = np.asarray(coefficients)
+        was_vector = vectors.ndim == 1
+        if was_vector:
+            vectors = vectors[:, None]
+        if vectors.ndim != 2 or vectors.shape[0] != self.npw:
+            raise ValueError("coefficients must have rank-local shape "
+                             "(nplane_local,) or (nplane_local, nvec)")
+        src = self.to_device(vectors, dtype)
+        dst = torch.empty_like(src)
+        self.device_apply(src, dst)
+        result = self.from_device(dst)
+        if out is not None:
+            target = np.asarray(out)
+            if was_vector and target.ndim == 1:
+                target = target[:, None]
+            if target.shape != result.shape or target.dtype != np.complex128:
+                raise ValueError("local-potential output has the wrong shape or dtype")
+            target[...] = result
+            return out
+        return result[:, 0] if was_vector else result
+
+
+def _upload_block(vectors, perm, npw, device, dtype):
+    if isinstance(vectors, torch.Tensor):
+        host = None
+        t = vectors
+        if t.dim() != 2 or t.shape[0] != npw:
+            raise ValueError("vectors must have shape (npw, nb)")
+        src = t.T.to(device)
+        if src.dtype not in (torch.complex128, torch.complex64):
+            src = src.to(torch.complex128)
+        src = src.contiguous()
+    else:
+        host = np.asarray(vectors)
+        if host.ndim != 2 or host.shape[0] != npw:
+            raise ValueError("vectors must have shape (npw, nb)")
+        src = torch.from_numpy(np.ascontiguousarray(host.T, dtype=np.complex128)).to(device)
+    out = torch.empty((src.shape[0], npw), dtype=dtype, device=device)
+    dv.permute_rows(0, perm, src, out, npw)
+    return out
+
+
+def _download_block(block, perm, npw):
+    ref = torch.empty((block.shape[0], npw), dtype=torch.complex128, device=block.device)
+    dv.permute_rows(1, perm, block, ref, npw)
+    return ref.cpu().numpy().T
+
+
+def _device_hamiltonian(operator, operator_into):
+    for fn in (operator_into, operator):
+        owner = getattr(fn, "__self__", None)
+        if isinstance(owner, PlaneWaveHamiltonian):
+            return owner
+    return None
+
+
+class _HostOperator:
+    """A non-device operator driven from device storage (identity row order)."""
+
+    def __init__(self, operator, operator_into, npw, device):
+        self.operator = operator
+        self.operator_into = operator_into
+        self.npw = npw
+        self.device = device
+
+    def device_apply(self, src, dst):
+        block = np.ascontiguousarray(src.cpu().numpy().T.astype(np.complex128), dtype=np.complex128)
+        block = np.asfortranarray(block)
+        out = np.empty_like(block, order="F")
+        if self.operator_into is not None:
+            self.operator_into(block, out)
+        else:
+            out[...] = self.operator(block)
+        dst.copy_(torch.from_numpy(np.ascontiguousarray(out.T)).to(dst.dtype))
+
+
+def _orthonormalize_device(block: torch.Tensor, npw: int, ws, threshold: float = 1.0e-12):
+    """Canonical orthonormalisation (``diagonalization.py:450-472``, serial):
+    Gram V^H V on the device, its small Hermitian eigensolve on the host,
+    V U s^{-1/2} (columns by decreasing Gram eigenvalue) on the device."""
+    n = int(block.shape[0])
+    device = block.device
+    if n == 0:
+        return block[:0].clone()
+    if n == 1:
+        sq = dv.column_squared_norms(block, npw)
+        if float(sq.cpu()[0]) == 0.0:
+            return block[:0].clone()
+        out = torch.empty_like(block)
+        dv.normalize_selected_columns(block, sq, torch.zeros(1, dtype=torch.int32, device=device),
+                                      out, npw)
+        return out
+    gram_t = torch.empty((n, n), dtype=block.dtype, device=device)
+    dv.gemm(nat.OP_C, n, n, npw, 1.0, block, block, 0.0, gram_t, ws)
+    gram = gram_t.cpu().numpy().T.astype(np.complex128)
+    gram = 0.5 * (gram + gram.conj().T)
+    squared, rotation = np.linalg.eigh(gram)
+    order = np.argsort(squared)[::-1]
+    squared = np.maximum(np.real(squared[order]), 0.0)
+    if not len(squared) or squared[0] == 0.0:
+        return block[:0].clone()
+    rank = int(np.count_nonzero(squared > threshold ** 2 * squared[0]))
+    transform = rotation[:, order[:rank]] / np.sqrt(squared[:rank])[None, :]
+    t_dev = torch.from_numpy(np.ascontiguousarray(transform.T)).to(device=device, dtype=block.dtype)
+    out = torch.empty((rank, npw), dtype=block.dtype, device=device)
+    dv.gemm(nat.OP_N, npw, rank, n, 1.0, block, t_dev, 0.0, out, ws)
+    return out
+
+
+def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, tolerance=1.0e-8,
+             max_iterations=40, subspace_multiplier=4, residual_factor=10.0,
+             residual_energy_scale=None, occupied_roots=None, initial_is_ritz=False,
+             initial_is_orthonormal=False, initial_eigenvalues=None, initial_applied=None,
+             mpi=None, global_dimension=None, global_row_indices=None, timers=None,
+             operator_into=None):
+    """Lowest eigenpairs by restarted block Davidson (QE ``cegterg``) on the
+    GPU; semantics of ``qepy_pw/diagonalization.py:527-880``."""
+    diagonal = np.asarray(diagonal, dtype=float)
+    mpi = mpi if mpi is not None else _SERIAL
+    if getattr(mpi, "size", 1) > 1:
+        raise ValueError("the device Davidson runs on one GPU; MPI size must be 1")
+    local_dimension = len(diagonal)
+    dimension = int(global_dimension) if global_dimension is not None else local_dimension
+    if not 1 <= number_of_roots <= dimension:
+        raise ValueError("number_of_roots must be between 1 and the matrix size")
+    if tolerance <= 0.0:
+        raise ValueError("Davidson tolerance must be positive")
+    if max_iterations < 1:
+        raise ValueError("Davidson max_iterations must be at least 1")
+    if subspace_multiplier < 2:
+        raise ValueError("diago_david_ndim must be at least 2")
+    if residual_factor is not None and residual_factor <= 0.0:
+        raise ValueError("Davidson residual_factor must be positive or None")
+    if residual_energy_scale is not None and residual_energy_scale <= 0.0:
+        raise ValueError("Davidson residual_energy_scale must be positive or None")
+    if occupied_roots is None:
+        occupied_roots = number_of_roots
+    if not 0 <= occupied_roots <= number_of_roots:
+        raise ValueError("occupied_roots must be between zero and number_of_roots")
+    root_tolerances = np.full(number_of_roots, max(5.0 * tolerance, 5.0e-6))
+    root_tolerances[:occupied_roots] = tolerance
+
+    if initial_vectors is None:
+        rows = (np.arange(local_dimension) if global_row_indices is None
+                else np.asarray(global_row_indices, dtype=int))
+        trial = np.zeros((local_dimension, number_of_roots), dtype=complex)
+        for column in range(number_of_roots):
+            matches = np.flatnonzero(rows == column)
+            if len(matches):
+                trial[matches[0], column] = 1.0
+    else:
+        trial = initial_vectors
+        shape = tuple(trial.shape)
+        if len(shape) != 2 or shape[0] != local_dimension or shape[1] < number_of_roots:
+            raise ValueError("initial_vectors must have shape "
+                             "(matrix size, at least number_of_roots)")
+
+    ham = _device_hamiltonian(operator, operator_into)
+    npw = local_dimension
+    if ham is not None:
+        if ham.npw != npw:
+            raise ValueError("diagonal length does not match the Hamiltonian basis")
+        device, perm_dev, perm = ham.device, ham.plan.perm_device, ham.plan.perm
+        op = ham
+        ws = ham.workspace
+    else:
+        device = dv.require_device()
+        perm_dev, perm = None, None
+        op = _HostOperator(operator, operator_into, npw, device)
+        ws = dv.Workspace(device)
+    dtype = torch.complex128
+    diag_dev = torch.from_numpy(np.ascontiguousarray(
+        diagonal if perm is None else diagonal[perm])).to(device)
+
+    trial_dev = _upload_block(trial, perm_dev, npw, device, dtype)
+    if initial_is_ritz or initial_is_orthonormal:
+        basis = trial_dev[:number_of_roots]
+    else:
+        with _measure(timers, "cegterg:over", device):
+            basis = _orthonormalize_device(trial_dev, npw, ws)
+    if basis.shape[0] < number_of_roots:
+        raise ValueError("initial Davidson vectors are linearly dependent")
+    maximum_subspace = min(dimension, max(basis.shape[0], 2 * number_of_roots,
+                                          subspace_multiplier * number_of_roots))
+    active = int(basis.shape[0])
+    V = torch.empty((maximum_subspace, npw), dtype=dtype, device=device)
+    HV = torch.empty_like(V)
+    V[:active].copy_(basis)
+    del basis, trial_dev
+    if initial_applied is None:
+        op.device_apply(V[:active], HV[:active])
+        happ = active
+    else:
+        shape = tuple(initial_applied.shape)
+        if shape != (npw, active):
+            raise ValueError("initial_applied must match the initial Ritz block")
+        HV[:active].copy_(_upload_block(initial_applied, perm_dev, npw, device, dtype))
+        happ = 0
+    projected = np.zeros((maximum_subspace, maximum_subspace), dtype=complex)
+    overlap = np.zeros_like(projected)
+
+    def update_projected(first):
+        with _measure(timers, "cegterg:over", device):
+            nnew = active - first
+            rows = torch.empty((2, active, nnew), dtype=dtype, device=device)
+            dv.gemm(nat.OP_C, nnew, active, npw, 1.0, V[first:active], HV[:active], 0.0,
+                    rows[0], ws)
+            dv.gemm(nat.OP_C, nnew, active, npw, 1.0, V[first:active], V[:active], 0.0,
+                    rows[1], ws)
+            host = rows.cpu().numpy()
+            h_rows = host[0].T
+            s_rows = host[1].T
+            projected[first:active, :active] = h_rows
+            overlap[first:active, :active] = s_rows
+            if first:
+                projected[:first, first:active] = h_rows[:, :first].conj().T
+                overlap[:first, first:active] = s_rows[:, :first].conj().T
+            sl = slice(first, active)
+            projected[sl, sl] = 0.5 * (projected[sl, sl] + projected[sl, sl].conj().T)
+            overlap[sl, sl] = 0.5 * (overlap[sl, sl] + overlap[sl, sl].conj().T)
+
+    update_projected(0)
+
+    def ritz_pairs():
+        with _measure(timers, "cdiaghg"):
+            roots, coefficients = _lowest_generalized_eigh(
+                projected[:active, :active], overlap[:active, :active], number_of_roots)
+        with _measure(timers, "cegterg:upda", device):
+            coef = torch.from_numpy(np.ascontiguousarray(coefficients.T)).to(device)
+            X = torch.empty((number_of_roots, npw), dtype=dtype, device=device)
+            R = torch.empty_like(X)
+            dv.gemm(nat.OP_N, npw, number_of_roots, active, 1.0, V[:active], coef, 0.0, X, ws)
+            dv.gemm(nat.OP_N, npw, number_of_roots, active, 1.0, HV[:active], coef, 0.0, R, ws)
+            e = torch.from_numpy(np.ascontiguousarray(roots, dtype=np.float64)).to(device)
+            dv.subtract_band_energies(R, X, e, npw)
+        return roots, X, R
+
+    if initial_is_ritz:
+        if initial_eigenvalues is None:
+            ip = dv.column_inner_products(V[:number_of_roots], HV[:number_of_roots], npw)
+            values = ip.cpu().numpy()[0::2].copy()
+        else:
+            values = np.asarray(initial_eigenvalues[:number_of_roots], dtype=float)
+        X = V[:number_of_roots].clone()
+        R = HV[:number_of_roots].clone()
+        dv.subtract_band_energies(R, X, torch.from_numpy(np.ascontiguousarray(values)).to(device),
+                                  npw)
+    else:
+        values, X, R = ritz_pairs()
+    track = residual_factor is not None or residual_energy_scale is not None
+
+    def residual_norms_of(block):
+        sq = dv.column_squared_norms(block, npw).cpu().numpy()
+        return np.sqrt(np.maximum(sq, 0.0))
+
+    residual_norms = residual_norms_of(R) if track else np.empty(number_of_roots)
+    completed = 0
+    number_unconverged = number_of_roots
+    unconverged = np.ones(number_of_roots, dtype=bool)
+    for iteration in range(1, max_iterations + 1):
+        completed = iteration
+        previous = values
+        with _measure(timers, "g_psi", device):
+            cols = np.flatnonzero(unconverged).astype(np.int32)
+            cols_dev = torch.from_numpy(cols).to(device)
+            e_dev = torch.from_numpy(np.ascontiguousarray(values[cols])).to(device)
+            corrections = torch.empty((len(cols), npw), dtype=dtype, device=device)
+            dv.qe_precondition(R, cols_dev, e_dev, diag_dev, corrections, npw)
+        # _normalize_columns (diagonalization.py:500-524): no orthogonalisation
+        sq_dev = dv.column_squared_norms(corrections, npw)
+        sq = sq_dev.cpu().numpy()
+        if not len(sq):
+            break
+        scale = max(float(np.max(sq)), 1.0e-300)
+        keep = np.flatnonzero(sq > (1.0e-14) ** 2 * scale).astype(np.int32)
+        if not len(keep):
+            break
+        old = active
+        active += len(keep)
+        dv.normalize_selected_columns(corrections, sq_dev, torch.from_numpy(keep).to(device),
+                                      V[old:active], npw)
+        del X, R, corrections
+        op.device_apply(V[old:active], HV[old:active])
+        happ += len(keep)
+        update_projected(old)
+        values, X, R = ritz_pairs()
+        if track:
+            residual_norms = residual_norms_of(R)
+        converged = np.abs(values - previous) < root_tolerances
+        if residual_factor is not None:
+            converged &= residual_norms < residual_factor * root_tolerances
+        if residual_energy_scale is not None:
+            converged &= residual_norms ** 2 < residual_energy_scale * root_tolerances
+        unconverged = ~converged
+        number_unconverged = int(np.count_nonzero(unconverged))
+        if number_unconverged == 0:
+            if not track:
+                residual_norms = residual_norms_of(R)
+            return DavidsonResult(values, _result_vectors(X, perm_dev, npw), residual_norms,
+                                  iteration, True, 0, happ)
+        if active + number_unconverged > maximum_subspace:
+            with _measure(timers, "cegterg:last", device):
+                active = number_of_roots
+                V[:active].copy_(X)
+                e = torch.from_numpy(np.ascontiguousarray(values)).to(device)
+                dv.restart_applied(X, R, e, HV[:active], npw)
+                projected[:active, :active] = 0.0
+                projected[np.arange(active), np.arange(active)] = values
+                overlap[:active, :active] = np.eye(active, dtype=complex)
+    if not track:
+        residual_norms = residual_norms_of(R)
+    return DavidsonResult(values, _result_vectors(X, perm_dev, npw), residual_norms, completed,
+                          False, number_unconverged, happ)
+
+
+def _result_vectors(X, perm_dev, npw):
+    if perm_dev is None:
+        return np.asfortranarray(X.to(torch.complex128).cpu().numpy().T)
+    return np.asfortranarray(_download_block(X, perm_dev, npw))
+
+
+def conjugate_gradient(*args, **kwargs):
+    """Band-by-band CG (reference ``diagonalization.py:993-1186``): not part
+    of the device path yet."""
+    raise NotImplementedError("conjugate_gradient is not implemented on the device path")
+
+
+def parallel_orbital(*args, **kwargs):
+    """ParO (reference ``diagonalization.py:1328-1467``): not part of the
+    device path yet."""
+    raise NotImplementedError("parallel_orbital is not implemented on the device path")
+
+
+def rmm_diis(*args, **kwargs):
+    """RMM-DIIS (reference ``diagonalization.py:1528-1756``): not part of the
+    device path yet."""
+    raise NotImplementedError("rmm_diis is not implemented on the device path")

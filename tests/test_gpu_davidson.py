@@ -1,0 +1,140 @@
+"""GPU parity of the device Davidson against dense brute force and the oracle.
+
+Bars (FP64 form): eigenvalues within 1e-10 Ry (5e-11 Ha) of the dense
+eigenvalues (small workload) or of the oracle Davidson (medium workload);
+residual norms consistent with an independent oracle h_psi; orthonormal
+Ritz vectors; the reference's error behaviour.
+"""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import davidson as odav
+from oracle.hpsi import OracleHamiltonian
+from qe_python_amd.synthetic import make_system
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).parent / "golden"
+TOL_HA = 0.5e-10
+
+
+def device_ham(s, **kw):
+    from qe_python_amd.diagonalization import PackedProjectorTerm, PlaneWaveHamiltonian
+    beta, coup, diag = s.packed_term()
+    return PlaneWaveHamiltonian(s.basis, s.potential_g, (PackedProjectorTerm(beta, coup, diag),),
+                                real_potential=s.real_potential, **kw)
+
+
+@pytest.fixture(scope="module")
+def small():
+    s = make_system("small")
+    return s, device_ham(s), np.load(GOLDEN / "small_hpsi.npz")["dense_eigenvalues"]
+
+
+def test_small_seven_roots(small):
+    from qe_python_amd.diagonalization import davidson
+    s, ham, exact = small
+    res = davidson(ham.apply, ham.diagonal, 7, initial_vectors=s.trial, tolerance=TOL_HA,
+                   operator_into=ham.apply_into)
+    assert res.converged
+    np.testing.assert_allclose(res.eigenvalues, exact[:7], atol=TOL_HA, rtol=0)
+    assert res.eigenvectors.shape == (s.npw, 7) and res.eigenvectors.flags.f_contiguous
+    gram = res.eigenvectors.conj().T @ res.eigenvectors
+    np.testing.assert_allclose(gram, np.eye(7), atol=1e-10)
+    bs = s.packed_term()
+    from types import SimpleNamespace
+    oh = OracleHamiltonian(s.basis.indices, s.basis.kinetic, s.shape, s.real_potential,
+                           (SimpleNamespace(beta=bs[0], coupling=bs[1], diagonal=bs[2]),))
+    r = oh.apply(res.eigenvectors) - res.eigenvectors * res.eigenvalues[None, :]
+    np.testing.assert_allclose(np.linalg.norm(r, axis=0), res.residual_norms, rtol=1e-4,
+                               atol=1e-11)
+    assert np.all(res.residual_norms < 10 * TOL_HA)
+
+
+def test_small_sixteen_roots_without_residual_test(small):
+    from qe_python_amd.diagonalization import davidson
+    s, ham, exact = small
+    res = davidson(ham.apply, ham.diagonal, 16, initial_vectors=s.trial, tolerance=TOL_HA,
+                   residual_factor=None, operator_into=ham.apply_into)
+    assert res.converged
+    np.testing.assert_allclose(res.eigenvalues, exact[:16], atol=TOL_HA, rtol=0)
+
+
+def test_matches_oracle_trajectory_closely(small):
+    from qe_python_amd.diagonalization import davidson
+    s, ham, _ = small
+    res = davidson(ham.apply, ham.diagonal, 7, initial_vectors=s.trial, tolerance=TOL_HA,
+                   operator_into=ham.apply_into)
+    from types import SimpleNamespace
+    bs = s.packed_term()
+    oh = OracleHamiltonian(s.basis.indices, s.basis.kinetic, s.shape, s.real_potential,
+                           (SimpleNamespace(beta=bs[0], coupling=bs[1], diagonal=bs[2]),),
+                           potential_average=float(np.real(s.potential_g.flat[0])))
+    ref = odav.davidson(oh.apply, oh.diagonal, 7, initial_vectors=s.trial, tolerance=TOL_HA,
+                        operator_into=oh.apply_into)
+    np.testing.assert_allclose(res.eigenvalues, ref.eigenvalues, atol=TOL_HA, rtol=0)
+    # identical algorithm, different rounding: iteration counts may differ by a few
+    assert abs(res.iterations - ref.iterations) <= 3
+
+
+def test_generic_operator_reference_pattern():
+    """tests/unit/core/test_diagonalization.py:8-31 pattern on the device solver."""
+    from qe_python_amd.diagonalization import davidson
+    rng = np.random.default_rng(7321)
+    a = rng.standard_normal((18, 18))
+    h = 0.5 * (a + a.T) + np.diag(np.linspace(1.0, 18.0, 18))
+    res = davidson(lambda x: h @ x, np.diag(h), 4, initial_vectors=np.eye(18, 6) + 0.01,
+                   tolerance=1e-11, max_iterations=60)
+    assert res.converged
+    np.testing.assert_allclose(res.eigenvalues, np.linalg.eigvalsh(h)[:4], atol=1e-9)
+    resid = h @ res.eigenvectors - res.eigenvectors * res.eigenvalues[None, :]
+    assert np.max(np.linalg.norm(resid, axis=0)) < 3e-6
+
+
+def test_errors_and_nonconvergence(small):
+    from qe_python_amd.diagonalization import davidson
+    s, ham, _ = small
+    with pytest.raises(ValueError):
+        davidson(ham.apply, ham.diagonal, 0)
+    with pytest.raises(ValueError):
+        davidson(ham.apply, ham.diagonal, 4, tolerance=-1.0)
+    with pytest.raises(ValueError):
+        davidson(ham.apply, ham.diagonal, 4, initial_vectors=s.trial[:, :2])
+    dep = np.zeros((s.npw, 4), dtype=complex)
+    dep[:, 0] = s.trial[:, 0]
+    with pytest.raises(ValueError, match="linearly dependent"):
+        davidson(ham.apply, ham.diagonal, 2, initial_vectors=dep)
+    res = davidson(ham.apply, ham.diagonal, 7, initial_vectors=s.trial, max_iterations=1,
+                   tolerance=TOL_HA)
+    assert not res.converged and res.iterations == 1 and res.number_unconverged > 0
+
+
+def test_initial_ritz_path(small):
+    from qe_python_amd.diagonalization import davidson
+    s, ham, exact = small
+    first = davidson(ham.apply, ham.diagonal, 7, initial_vectors=s.trial, tolerance=1e-6,
+                     residual_factor=None, operator_into=ham.apply_into)
+    applied = ham.apply(first.eigenvectors)
+    res = davidson(ham.apply, ham.diagonal, 7, initial_vectors=first.eigenvectors,
+                   initial_is_ritz=True, initial_applied=applied,
+                   initial_eigenvalues=first.eigenvalues, tolerance=TOL_HA,
+                   operator_into=ham.apply_into)
+    assert res.converged
+    assert res.hamiltonian_applications < first.hamiltonian_applications + 200
+    np.testing.assert_allclose(res.eigenvalues, exact[:7], atol=TOL_HA, rtol=0)
+
+
+def test_medium_workload_against_oracle_eigenvalues():
+    from qe_python_amd.diagonalization import davidson
+    golden = GOLDEN / "medium_davidson.npz"
+    if not golden.exists():
+        pytest.skip("medium oracle fixture not generated")
+    data = np.load(golden)
+    s = make_system("medium")
+    ham = device_ham(s)
+    res = davidson(ham.apply, ham.diagonal, 128, initial_vectors=s.trial, tolerance=TOL_HA,
+                   residual_factor=None, operator_into=ham.apply_into)
+    assert res.converged
+    np.testing.assert_allclose(res.eigenvalues, data["eigenvalues"], atol=TOL_HA, rtol=0)

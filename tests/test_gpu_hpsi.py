@@ -1,0 +1,170 @@
+"""GPU parity of h_psi (``PlaneWaveHamiltonian.apply``) against the oracle.
+
+Bars: FP64 h_psi relative error <= 1e-12 against the oracle (Frobenius norm
+over the block); FP32 (complex64 FFT / V psi / projections) relative error
+<= 2e-6.  Also: the reference's V_loc known answer (2e-14), Hermiticity,
+identity-potential round trip, term-type equivalence, device
+prepare_potential, layouts and the reference's error behaviour.
+"""
+
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+import torch
+
+from oracle.dense import potential_matrix
+from oracle.hpsi import OracleHamiltonian, apply_local, linear_slots
+from qe_python_amd.synthetic import make_system, phase_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def device_ham(s, terms=None, **kw):
+    from qe_python_amd.diagonalization import PackedProjectorTerm, PlaneWaveHamiltonian
+    if terms is None:
+        beta, coup, diag = s.packed_term()
+        terms = (PackedProjectorTerm(beta, coup, diag),)
+    return PlaneWaveHamiltonian(s.basis, s.potential_g, terms, real_potential=s.real_potential,
+                                **kw)
+
+
+def oracle_ham(s):
+    beta, coup, diag = s.packed_term()
+    term = SimpleNamespace(beta=beta, coupling=coup, diagonal=diag)
+    return OracleHamiltonian(s.basis.indices, s.basis.kinetic, s.shape, s.real_potential,
+                             (term,), potential_average=float(np.real(s.potential_g.flat[0])))
+
+
+@pytest.fixture(scope="module")
+def small():
+    return make_system("small")
+
+
+@pytest.fixture(scope="module")
+def medium():
+    return make_system("medium")
+
+
+def test_known_answer_on_device():
+    """tests/unit/core/test_basis.py:76-89 of the reference, through the
+    device prepare_potential (inverse_real) and V_loc pipeline."""
+    from qe_python_amd.diagonalization import PlaneWaveHamiltonian
+    shape = (6, 6, 6)
+    grid = np.indices(shape).sum(axis=0)
+    vr = 0.3 + 0.2 * np.cos(2.0 * np.pi * grid / shape[0])
+    vg = np.fft.fftn(vr) / np.prod(shape)
+    idx = np.asarray([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0]], dtype=np.int32)
+    c = np.asarray([1.0 + 0.2j, -0.3j, 0.4, -0.1 + 0.5j])
+    basis = SimpleNamespace(indices=idx, kinetic=np.zeros(4))
+    ham = PlaneWaveHamiltonian(basis, vg)
+    np.testing.assert_allclose(ham.apply(c), potential_matrix(vg, idx) @ c, atol=2.0e-14)
+
+
+def test_device_prepare_potential_matches_numpy(small):
+    from qe_python_amd.diagonalization import PlaneWaveHamiltonian
+    ham = PlaneWaveHamiltonian(small.basis, small.potential_g)
+    expect = np.real(np.fft.ifftn(small.potential_g)) * small.potential_g.size
+    np.testing.assert_allclose(ham.real_potential, expect, atol=1e-14, rtol=0)
+
+
+def test_small_hpsi_fp64_parity(small):
+    ham = device_ham(small)
+    got = ham.apply(small.trial)
+    ref = oracle_ham(small).apply(small.trial)
+    assert rel(got, ref) <= 1e-12
+
+
+def test_small_hpsi_against_golden(small):
+    from pathlib import Path
+    data = np.load(Path(__file__).parent / "golden" / "small_hpsi.npz")
+    got = device_ham(small).apply(data["trial4"])
+    assert rel(got, data["hpsi"]) <= 1e-12
+
+
+def test_medium_hpsi_fp64_parity(medium):
+    ham = device_ham(medium)
+    x = medium.trial[:, :24]
+    got = ham.apply(x)
+    ref = oracle_ham(medium).apply(x)
+    assert rel(got, ref) <= 1e-12
+
+
+def test_medium_hpsi_fp32_parity(medium):
+    ham = device_ham(medium, precision="fp32")
+    x = medium.trial[:, :16]
+    got = ham.apply(x)
+    ref = oracle_ham(medium).apply(x)
+    assert rel(got, ref) <= 2e-6
+
+
+def test_cells_54_64_hpsi_parity():
+    for name in ("cell54", "cell64"):
+        s = make_system(name, kpoint=np.array([0.21, 0.07, 0.33]))
+        x = s.trial[:, :6]
+        assert rel(device_ham(s).apply(x), oracle_ham(s).apply(x)) <= 1e-12, name
+
+
+def test_hermiticity_on_device(small):
+    ham = device_ham(small)
+    x, y = small.trial[:, :4], small.trial[:, 4:9]
+    np.testing.assert_allclose(x.conj().T @ ham.apply(y), ham.apply(x).conj().T @ y,
+                               rtol=1e-12, atol=1e-12)
+
+
+def test_identity_potential_round_trip(small):
+    from qe_python_amd.diagonalization import PlaneWaveHamiltonian
+    ham = PlaneWaveHamiltonian(small.basis, small.potential_g, (),
+                               real_potential=np.ones(small.shape))
+    x = small.trial[:, :5]
+    np.testing.assert_allclose(ham.apply(x), (small.basis.kinetic + 1.0)[:, None] * x,
+                               atol=1e-13, rtol=0)
+
+
+def test_term_types_agree_and_diagonal(small):
+    from qe_python_amd.diagonalization import FactorizedProjectorTerm
+    pos = small.projectors.positions
+    coupling = small.projectors.couplings[0]
+    beta_s = small.projectors.beta_species
+    fact = FactorizedProjectorTerm(beta_s, phase_matrix(small.basis.vectors, pos), coupling)
+    mats = tuple((beta_s * np.exp(-1j * (small.basis.vectors @ p))[:, None], coupling)
+                 for p in pos)
+    x = small.trial[:, :6]
+    hf, hm = device_ham(small, (fact,)), device_ham(small, mats)
+    of = OracleHamiltonian(small.basis.indices, small.basis.kinetic, small.shape,
+                           small.real_potential, (fact,))
+    assert rel(hf.apply(x), of.apply(x)) <= 1e-12
+    assert rel(hm.apply(x), of.apply(x)) <= 1e-12
+    np.testing.assert_allclose(hf.diagonal - hf.potential_average,
+                               of.diagonal - of.potential_average, atol=1e-12)
+    np.testing.assert_allclose(device_ham(small).diagonal, oracle_ham(small).diagonal,
+                               atol=1e-12)
+
+
+def test_native_layout_and_apply_into_and_torch_input(small):
+    from qe_python_amd.diagonalization import PackedProjectorTerm, PlaneWaveHamiltonian
+    beta, coup, diag = small.packed_term()
+    native = np.ascontiguousarray(np.transpose(small.real_potential, (2, 0, 1)))
+    ham = PlaneWaveHamiltonian(small.basis, small.potential_g,
+                               (PackedProjectorTerm(beta, coup, diag),),
+                               real_potential=native, native_potential_layout=True)
+    x = np.asfortranarray(small.trial[:, :3])
+    ref = oracle_ham(small).apply(x)
+    out = np.empty_like(x, order="F")
+    assert ham.apply_into(x, out) is out
+    assert rel(out, ref) <= 1e-12
+    assert rel(ham.apply(x[:, 1]), ref[:, 1]) <= 1e-12
+    xt = torch.from_numpy(x).to("cuda")
+    assert rel(ham.apply(xt).cpu().numpy(), ref) <= 1e-12
+
+
+def test_shape_errors_are_value_errors(small):
+    ham = device_ham(small)
+    with pytest.raises(ValueError):
+        ham.apply(np.zeros((small.npw + 1, 2), dtype=complex))
+    with pytest.raises(ValueError):
+        ham.apply_into(small.trial[:, :2], np.zeros((small.npw, 3), dtype=complex))
