@@ -78,8 +78,8 @@ const int32_t* qepw_plan_device_permutation(const qepw_plan* plan);
  * Replaces: apply_serial (qepy_pw/_native_fft.pyx:2338-2455) and
  *   apply_serial_spatial (:2458-2534, C :58-135), called from
  *   LocalPotentialWorkspace.apply (qepy_pw/basis.py:1354-1455).
- * potential: device real (n3, n1, n2) plane-major grid (the reference's
- *   "native" layout), double for QEPW_C128, float for QEPW_C64.
+ * potential: device real (n3, n2, n1) grid (plane z, row y, x fastest),
+ *   double for QEPW_C128, float for QEPW_C64.
  * kinetic: device real (npw) in device order, same type, or NULL.
  * vectors / result: device complex, column-major, leading dims ldv / ldr.
  * workspace: at least qepw_apply_local_workspace(plan, precision, 1) bytes;
@@ -93,7 +93,7 @@ int qepw_apply_local(const qepw_plan* plan, int precision, const void* vectors,
 
 /*
  * out[z, x, y] = Re sum_G V(G) exp(+i G.r) (unnormalised backward FFT) of a
- * C-order (n1, n2, n3) complex128 grid, written in (n3, n1, n2) layout.
+ * C-order (n1, n2, n3) complex128 grid, written in (n3, n2, n1) layout.
  * Replaces: inverse_real (qepy_pw/_native_fft.pyx:2188-2202) as used by
  *   LocalPotentialWorkspace.prepare_potential (qepy_pw/basis.py:777-792).
  */

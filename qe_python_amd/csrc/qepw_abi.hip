@@ -77,7 +77,9 @@ void free_tables(qepw::StickTables& t) {
   hipFree(t.d_gpos);
   hipFree(t.d_plane);
   hipFree(t.d_xcols);
-  t.d_off = t.d_gpos = t.d_plane = t.d_xcols = nullptr;
+  hipFree(t.d_stickq);
+  hipFree(t.d_colmap);
+  t.d_off = t.d_gpos = t.d_plane = t.d_xcols = t.d_stickq = t.d_colmap = nullptr;
 }
 
 void free_plan(qepw::Plan& p) {
@@ -94,7 +96,7 @@ void free_plan(qepw::Plan& p) {
 }
 
 struct TableHost {
-  std::vector<int> off, gpos, plane, xcols;
+  std::vector<int> off, gpos, plane, xcols, stickq, colmap;
 };
 
 hipError_t upload_tables(qepw::StickTables& t, const TableHost& h) {
@@ -102,6 +104,8 @@ hipError_t upload_tables(qepw::StickTables& t, const TableHost& h) {
   if ((e = upload(&t.d_off, h.off)) != hipSuccess) return e;
   if ((e = upload(&t.d_gpos, h.gpos)) != hipSuccess) return e;
   if ((e = upload(&t.d_plane, h.plane)) != hipSuccess) return e;
+  if ((e = upload(&t.d_stickq, h.stickq)) != hipSuccess) return e;
+  if ((e = upload(&t.d_colmap, h.colmap)) != hipSuccess) return e;
   return upload(&t.d_xcols, h.xcols);
 }
 
@@ -168,6 +172,13 @@ int qepw_plan_create(const int32_t* miller, int64_t npw, int n1, int n2, int n3,
   sphere.off.push_back((int)npw);
   for (int x = 0; x < n1; ++x)
     if (xs_seen[x]) sphere.xcols.push_back(x);
+  sphere.colmap.assign(n1, -1);
+  for (size_t c = 0; c < sphere.xcols.size(); ++c) sphere.colmap[sphere.xcols[c]] = (int)c;
+  sphere.stickq.resize(nst);
+  for (int s = 0; s < nst; ++s) {
+    const int x = sphere.plane[s] / n2, y = sphere.plane[s] % n2;
+    sphere.stickq[s] = sphere.colmap[x] * n2 + y;
+  }
 
   TableHost full;
   const int nfull = n1 * n2;

@@ -18,6 +18,7 @@
 // Only active sticks (19% of the 72^3 grid at 30 Ry) travel through HBM.
 #include "qepw_internal.h"
 #include "qepw_fft_lds.h"
+#include "qepw_regfft.h"
 
 namespace qepw {
 
@@ -116,17 +117,151 @@ __global__ __launch_bounds__(kPlaneThreads) void k_planes(
   lds_fft<T, kPlaneThreads, kPlanePoints>(P, nxcols, n2, 1, xcols, p2, tw2, +1, tid);
   // backward along x (stride n2) for every y row
   lds_fft<T, kPlaneThreads, kPlanePoints>(P, n2, 1, n2, nullptr, p1, tw1, +1, tid);
+  // V and the real output use the (n3, n2, n1) layout: element (x, y) of
+  // plane z at z*np + y*n1 + x; the LDS plane holds (x, y) at x*n2 + y.
   if (MODE == kPlaneReal) {
     T* o = real_out + (int64_t)z * np;
-    for (int i = tid; i < np; i += kPlaneThreads) o[i] = P[i].x;
+    for (int i = tid; i < np; i += kPlaneThreads) {
+      const int y = i / n1, x = i - y * n1;
+      o[i] = P[x * n2 + y].x;
+    }
     return;
   }
   const T* v = V + (int64_t)z * np;
-  for (int i = tid; i < np; i += kPlaneThreads) P[i] = cscale(P[i], v[i]);
+  for (int i = tid; i < np; i += kPlaneThreads) {
+    const int x = i / n2, y = i - x * n2;
+    P[i] = cscale(P[i], v[y * n1 + x]);
+  }
   __syncthreads();
   lds_fft<T, kPlaneThreads, kPlanePoints>(P, n2, 1, n2, nullptr, p1, tw1, -1, tid);
   lds_fft<T, kPlaneThreads, kPlanePoints>(P, nxcols, n2, 1, xcols, p2, tw2, -1, tid);
   for (int s = tid; s < nst; s += kPlaneThreads) w[s] = P[plane_of[s]];
+}
+
+// Fast xy-plane kernel for square planes (n1 = n2 = N): the LDS holds only
+// the nxc active x columns of the plane, Q[c * N + y]; every 1-D FFT is a
+// register "team" FFT (qepw_regfft.h) and the row pass fuses the backward
+// x-FFT, the V(r) multiply and the forward x-FFT without leaving registers
+// except for the team exchanges.  Four workgroup barriers per plane.
+constexpr int kTeamThreads = 256;
+
+template <int N> struct TeamGeom {
+  static constexpr int L = team_L(N);
+  static constexpr int M = N / L;
+  static constexpr int per_wave = 64 / M;
+  static constexpr int teams = (kTeamThreads / 64) * per_wave;
+};
+
+template <typename T, int N>
+__global__ __launch_bounds__(kTeamThreads) void k_planes_team(
+    cplx<T>* __restrict__ work, int64_t band_stride, int nst,
+    const int* __restrict__ stick_q, const int* __restrict__ colmap, int nxc,
+    const T* __restrict__ V, const cplx<T>* __restrict__ tw) {
+  using G = TeamGeom<N>;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* Q = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* scratch_all = Q + nxc * N;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int team_in_wave = lane / G::M;
+  const int t = lane - team_in_wave * G::M;
+  const bool in_team = team_in_wave < G::per_wave;
+  const int team = wave * G::per_wave + team_in_wave;
+  cplx<T>* S = scratch_all + team * N;
+  const int z = blockIdx.x;
+  const int64_t b = blockIdx.y;
+  const int nq = nxc * N;
+  const cplx<T> zero = mk<T>(T(0), T(0));
+  cplx<T>* w = work + b * band_stride + (int64_t)z * nst;
+
+  for (int i = tid; i < nq; i += kTeamThreads) Q[i] = zero;
+  __syncthreads();
+  for (int s = tid; s < nst; s += kTeamThreads) Q[stick_q[s]] = w[s];
+  __syncthreads();
+
+  // backward y-FFT of every active column (in place in Q)
+  if (in_team) {
+    for (int c = team; c < nxc; c += G::teams) {
+      cplx<T>* col = Q + c * N;
+      team_fft<T, N, +1>(t, S, tw, [&](int j) { return col[j]; },
+                         [&](int k, cplx<T> v) { col[k] = v; });
+    }
+  }
+  __syncthreads();
+
+  // rows: backward x-FFT (input only at active columns), V, forward x-FFT
+  // (output only at active columns)
+  if (in_team) {
+    const T* vz = V + (int64_t)z * N * N;
+    for (int y = team; y < N; y += G::teams) {
+      const T* vrow = vz + y * N;
+      team_fft<T, N, +1>(
+          t, S, tw,
+          [&](int j) {
+            const int c = colmap[j];
+            return c >= 0 ? Q[c * N + y] : zero;
+          },
+          [&](int k, cplx<T> v) { S[k] = cscale(v, vrow[k]); });
+      // S now holds V psi of this row in natural order (written after the
+      // team's last exchange read, read back before any exchange write)
+      wave_lds_sync();
+      team_fft<T, N, -1>(
+          t, S, tw, [&](int j) { return S[j]; },
+          [&](int k, cplx<T> v) {
+            const int c = colmap[k];
+            if (c >= 0) Q[c * N + y] = v;
+          });
+    }
+  }
+  __syncthreads();
+
+  // forward y-FFT of every active column
+  if (in_team) {
+    for (int c = team; c < nxc; c += G::teams) {
+      cplx<T>* col = Q + c * N;
+      team_fft<T, N, -1>(t, S, tw, [&](int j) { return col[j]; },
+                         [&](int k, cplx<T> v) { col[k] = v; });
+    }
+  }
+  __syncthreads();
+  for (int s = tid; s < nst; s += kTeamThreads) w[s] = Q[stick_q[s]];
+}
+
+// Square plane sizes with an instantiated fast kernel.
+#define QEPW_TEAM_SIZES(X) \
+  X(16) X(18) X(20) X(24) X(27) X(30) X(32) X(36) X(40) X(45) X(48) X(50) X(54) \
+  X(60) X(64) X(72) X(80) X(81) X(90) X(96)
+
+template <typename T>
+static size_t team_lds_bytes(int n, int nxc) {
+  size_t teams = 0;
+#define QEPW_TEAMS_OF(NN) if (n == NN) teams = TeamGeom<NN>::teams;
+  QEPW_TEAM_SIZES(QEPW_TEAMS_OF)
+#undef QEPW_TEAMS_OF
+  return ((size_t)nxc * n + teams * n) * sizeof(cplx<T>);
+}
+
+// Launches the fast plane kernel if one exists for this plane; returns false
+// otherwise (the generic kernel is used).
+template <typename T>
+static bool launch_planes_team(const Plan& p, const StickTables& st, cplx<T>* work,
+                               int64_t bstride, const T* V, const cplx<T>* tw, int nb,
+                               hipStream_t stream) {
+  if (p.n1 != p.n2 || !st.d_stickq || !st.d_colmap) return false;
+  const size_t lds = team_lds_bytes<T>(p.n1, st.nxcols);
+  if (lds == 0 || lds > (size_t)kMaxLdsBytes) return false;
+  const dim3 grid(p.n3, nb);
+#define QEPW_LAUNCH_TEAM(NN)                                                              \
+  if (p.n1 == NN) {                                                                       \
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_planes_team<T, NN>),            \
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);            \
+    hipLaunchKernelGGL((k_planes_team<T, NN>), grid, dim3(kTeamThreads), lds, stream,     \
+                       work, bstride, st.nst, st.d_stickq, st.d_colmap, st.nxcols, V, tw); \
+    return true;                                                                          \
+  }
+  QEPW_TEAM_SIZES(QEPW_LAUNCH_TEAM)
+#undef QEPW_LAUNCH_TEAM
+  return false;
 }
 
 template <typename T> const cplx<T>* twiddles(const Plan& p, int axis);
@@ -158,6 +293,7 @@ static void set_lds_attrs(size_t stick_bytes, size_t plane_bytes) {
                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)stick_bytes);
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sticks_forward<T>),
                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)stick_bytes);
+  if (plane_bytes == 0 || plane_bytes > (size_t)kMaxLdsBytes) return;
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_planes<T, kPlaneApply>),
                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_bytes);
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_planes<T, kPlaneReal>),
@@ -170,12 +306,24 @@ size_t apply_local_band_bytes(const Plan& p, int precision) {
 }
 
 template <typename T>
+static bool team_available(const Plan& p, const StickTables& st) {
+  if (p.n1 != p.n2 || !st.d_stickq || !st.d_colmap) return false;
+  const size_t lds = team_lds_bytes<T>(p.n1, st.nxcols);
+  return lds > 0 && lds <= (size_t)kMaxLdsBytes;
+}
+
+template <typename T>
 int apply_local_t(const Plan& p, const void* vectors, int64_t ldv, int nb,
                   const void* potential, const void* kinetic, void* result,
                   int64_t ldr, void* workspace, size_t ws_bytes, hipStream_t stream) {
   const StickTables& st = p.sphere;
-  int rc = check_fft_limits<T>(p, st);
-  if (rc) return rc;
+  const bool team = team_available<T>(p, st);
+  if (st.group * p.n3 > lds_fft_capacity(p.line[2], kStickThreads, kStickPts))
+    return QEPW_ERR_UNSUPPORTED;
+  if (!team) {
+    int rc = check_fft_limits<T>(p, st);
+    if (rc) return rc;
+  }
   if (nb <= 0) return QEPW_OK;
   const size_t band_bytes = (size_t)p.n3 * st.nst * sizeof(cplx<T>);
   int chunk = (int)(ws_bytes / band_bytes);
@@ -183,7 +331,7 @@ int apply_local_t(const Plan& p, const void* vectors, int64_t ldv, int nb,
   if (chunk > 65535) chunk = 65535;
   const size_t stick_lds = (size_t)st.group * p.n3 * sizeof(cplx<T>);
   const size_t plane_lds = (size_t)p.n1 * p.n2 * sizeof(cplx<T>);
-  set_lds_attrs<T>(stick_lds, plane_lds);
+  set_lds_attrs<T>(stick_lds, team ? 0 : plane_lds);
   const int64_t bstride = (int64_t)p.n3 * st.nst;
   const T inv_n = T(1.0 / ((double)p.n1 * p.n2 * p.n3));
   cplx<T>* work = static_cast<cplx<T>*>(workspace);
@@ -195,10 +343,12 @@ int apply_local_t(const Plan& p, const void* vectors, int64_t ldv, int nb,
     hipLaunchKernelGGL(k_sticks_inverse<T>, gs, dim3(kStickThreads), stick_lds, stream,
                        src, ldv, work, bstride, st.nst, p.n3, st.d_off, st.d_gpos,
                        st.group, p.line[2], twiddles<T>(p, 2));
-    hipLaunchKernelGGL((k_planes<T, kPlaneApply>), gp, dim3(kPlaneThreads), plane_lds, stream,
-                       work, bstride, st.nst, st.d_plane, st.d_xcols, st.nxcols,
-                       static_cast<const T*>(potential), static_cast<T*>(nullptr), p.n1, p.n2,
-                       p.line[0], p.line[1], twiddles<T>(p, 0), twiddles<T>(p, 1));
+    if (!team || !launch_planes_team<T>(p, st, work, bstride, static_cast<const T*>(potential),
+                                        twiddles<T>(p, 0), nbc, stream))
+      hipLaunchKernelGGL((k_planes<T, kPlaneApply>), gp, dim3(kPlaneThreads), plane_lds,
+                         stream, work, bstride, st.nst, st.d_plane, st.d_xcols, st.nxcols,
+                         static_cast<const T*>(potential), static_cast<T*>(nullptr), p.n1,
+                         p.n2, p.line[0], p.line[1], twiddles<T>(p, 0), twiddles<T>(p, 1));
     hipLaunchKernelGGL(k_sticks_forward<T>, gs, dim3(kStickThreads), stick_lds, stream,
                        work, bstride, src, ldv, static_cast<const T*>(kinetic), dst, ldr,
                        st.nst, p.n3, st.d_off, st.d_gpos, st.group, p.line[2],
@@ -224,7 +374,7 @@ size_t inverse_real_bytes(const Plan& p) {
 }
 
 // V(r) = Re backward-FFT(V(G)) of a C-order (n1, n2, n3) complex grid, written
-// in the plane-major (n3, n1, n2) layout the plane kernel reads
+// in the (n3, n2, n1) layout the plane kernels read
 // (reference: _native_fft.pyx:2188-2202 inverse_real; basis.py:777-792).
 int inverse_real(const Plan& p, const void* grid_g, void* out_real, void* workspace,
                  size_t ws_bytes, hipStream_t stream) {

@@ -28,6 +28,8 @@ struct StickTables {
   int* d_gpos = nullptr;   // npoints: (stick * n3 + z) of each row
   int* d_plane = nullptr;  // nst: x * n2 + y of each stick
   int* d_xcols = nullptr;  // nxcols: active x values
+  int* d_stickq = nullptr;  // nst: c * n2 + y, c = rank of the stick's x in xcols
+  int* d_colmap = nullptr;  // n1: rank of x among active columns, or -1
 };
 
 struct Plan {

@@ -1,0 +1,184 @@
+// Register-resident DFTs and the "team" line FFT used by the xy-plane kernel.
+//
+// RegDft<T, M, SIGN>: an M-point DFT (M 2,3,5-smooth, <= 16) held in one
+// lane's registers, X[k] = sum_j v[j] exp(SIGN 2 pi i jk/M), computed by
+// recursive decimation in time M = R*S:
+//   A_j1[k1] = DFT_S over j2 of v[j1 + R j2];
+//   X[k1 + S k2] = DFT_R over j1 of (w_M^(j1 k1) A_j1[k1]).
+// Twiddles are compile-time constants (a constexpr sine/cosine evaluated by
+// the compiler from their Taylor series after exact octant reduction).
+//
+// team_fft<T, N, SIGN>: one line of length N = L*M (M >= L, both <= 16) by a
+// team of M lanes of one wave: lanes j1 < L take the M-point DFTs of the
+// decimated sequences x[j1 + L j2], multiply by w_N^(j1 k1) and exchange
+// through the team's LDS scratch; lanes k1 < M then take the L-point DFTs
+// that give X[k1 + M k2].  The exchange needs no workgroup barrier: the team
+// lives in one wave, whose LDS operations complete in program order.
+#pragma once
+
+#include "qepw_fft_lds.h"
+
+namespace qepw {
+
+// ---- compile-time unit roots -------------------------------------------
+constexpr double kPiD = 3.14159265358979323846264338327950288;
+
+constexpr double cx_sin_small(double x) {  // |x| <= pi/4
+  double term = x, sum = x;
+  for (int i = 1; i < 14; ++i) {
+    term *= -x * x / ((2.0 * i) * (2.0 * i + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cx_cos_small(double x) {  // |x| <= pi/4
+  double term = 1.0, sum = 1.0;
+  for (int i = 1; i < 14; ++i) {
+    term *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
+    sum += term;
+  }
+  return sum;
+}
+// cos / sin of 2 pi k / n with k reduced exactly into an octant.
+constexpr double cx_cos2pi(int k, int n) {
+  k %= n;
+  if (k < 0) k += n;
+  // theta = 2 pi k / n = (pi/4) * (8k/n); octant o = floor(8k/n)
+  const int o = (8 * k) / n;
+  const double r = 2.0 * kPiD * (double)(8 * k - o * n) / (8.0 * n);  // in [0, pi/4)
+  switch (o) {
+    case 0: return cx_cos_small(r);
+    case 1: return cx_sin_small(kPiD / 4 - r);
+    case 2: return -cx_sin_small(r);
+    case 3: return -cx_cos_small(kPiD / 4 - r);
+    case 4: return -cx_cos_small(r);
+    case 5: return -cx_sin_small(kPiD / 4 - r);
+    case 6: return cx_sin_small(r);
+    default: return cx_cos_small(kPiD / 4 - r);
+  }
+}
+constexpr double cx_sin2pi(int k, int n) {
+  k %= n;
+  if (k < 0) k += n;
+  const int o = (8 * k) / n;
+  const double r = 2.0 * kPiD * (double)(8 * k - o * n) / (8.0 * n);
+  switch (o) {
+    case 0: return cx_sin_small(r);
+    case 1: return cx_cos_small(kPiD / 4 - r);
+    case 2: return cx_cos_small(r);
+    case 3: return cx_sin_small(kPiD / 4 - r);
+    case 4: return -cx_sin_small(r);
+    case 5: return -cx_cos_small(kPiD / 4 - r);
+    case 6: return -cx_cos_small(r);
+    default: return -cx_sin_small(kPiD / 4 - r);
+  }
+}
+
+template <int M> struct UnitRoots {
+  // w[k] = exp(+2 pi i k / M)
+  double c[M > 0 ? M : 1];
+  double s[M > 0 ? M : 1];
+  constexpr UnitRoots() : c(), s() {
+    for (int k = 0; k < M; ++k) {
+      c[k] = cx_cos2pi(k, M);
+      s[k] = cx_sin2pi(k, M);
+    }
+  }
+};
+
+constexpr int smallest_radix(int m) {
+  return (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : 5;
+}
+
+template <typename T, int M, int SIGN> struct RegDft {
+  static __device__ __forceinline__ void run(cplx<T>* v) {
+    constexpr int R = smallest_radix(M);
+    constexpr int S = M / R;
+    constexpr UnitRoots<M> w{};
+    cplx<T> a[R][S];
+#pragma unroll
+    for (int j1 = 0; j1 < R; ++j1)
+#pragma unroll
+      for (int j2 = 0; j2 < S; ++j2) a[j1][j2] = v[j1 + R * j2];
+#pragma unroll
+    for (int j1 = 0; j1 < R; ++j1) RegDft<T, S, SIGN>::run(a[j1]);
+#pragma unroll
+    for (int k1 = 0; k1 < S; ++k1) {
+      cplx<T> b[R];
+#pragma unroll
+      for (int j1 = 0; j1 < R; ++j1) {
+        const int e = (j1 * k1) % M;
+        if (e == 0) {
+          b[j1] = a[j1][k1];
+        } else {
+          cplx<T> tw;
+          tw.x = (T)w.c[e];
+          tw.y = (T)(SIGN > 0 ? w.s[e] : -w.s[e]);
+          b[j1] = cmul(a[j1][k1], tw);
+        }
+      }
+      SmallDft<T, R>::run(b, SIGN);
+#pragma unroll
+      for (int k2 = 0; k2 < R; ++k2) v[k1 + S * k2] = b[k2];
+    }
+  }
+};
+
+template <typename T, int SIGN> struct RegDft<T, 1, SIGN> {
+  static __device__ __forceinline__ void run(cplx<T>*) {}
+};
+
+constexpr int team_L(int n) {
+  int best = 0;
+  for (int l = 1; l <= 16; ++l)
+    if (n % l == 0 && n / l <= 16 && n / l >= l) best = l;
+  return best;
+}
+
+// Orders this wave's LDS accesses around a team exchange (no workgroup sync).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One N-point line by the M lanes of a team; ``t`` is the lane's index in
+// the team; ``scratch`` holds N elements private to the team; ``tw`` is the
+// w_N^k = exp(-2 pi i k / N) table.  load(j) gives input j, store(k, v)
+// receives output k (each exactly once, by lane k % M).
+template <typename T, int N, int SIGN, class Load, class Store>
+__device__ __forceinline__ void team_fft(int t, cplx<T>* scratch,
+                                         const cplx<T>* __restrict__ tw, Load load,
+                                         Store store) {
+  constexpr int L = team_L(N);
+  constexpr int M = N / L;
+  static_assert(L >= 1 && M >= L && M <= 16, "unsupported team FFT length");
+  if (t < L) {
+    cplx<T> a[M];
+#pragma unroll
+    for (int j2 = 0; j2 < M; ++j2) a[j2] = load(t + L * j2);
+    RegDft<T, M, SIGN>::run(a);
+#pragma unroll
+    for (int k1 = 0; k1 < M; ++k1) {
+      cplx<T> v = a[k1];
+      if (k1 > 0 && t > 0) {
+        cplx<T> w = tw[t * k1];
+        if (SIGN > 0) w.y = -w.y;
+        v = cmul(v, w);
+      }
+      scratch[t * M + k1] = v;
+    }
+  }
+  wave_lds_sync();
+  if (t < M) {
+    cplx<T> b[L];
+#pragma unroll
+    for (int j1 = 0; j1 < L; ++j1) b[j1] = scratch[j1 * M + t];
+    RegDft<T, L, SIGN>::run(b);
+#pragma unroll
+    for (int k2 = 0; k2 < L; ++k2) store(t + M * k2, b[k2]);
+  }
+  wave_lds_sync();
+}
+
+}  // namespace qepw

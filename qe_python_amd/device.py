@@ -218,12 +218,12 @@ class DeviceFFTPlan:
         nat.check(rc, "qepw_apply_local")
 
     def inverse_real(self, potential_g: torch.Tensor, ws: Workspace) -> torch.Tensor:
-        """(n3, n1, n2) float64 = Re backward FFT of a C-order complex grid."""
+        """(n3, n2, n1) float64 = Re backward FFT of a C-order complex grid."""
         if potential_g.dtype != C128 or tuple(potential_g.shape) != self.shape:
             raise ValueError("potential_g must be a complex128 grid of the plan's shape")
         g = potential_g.contiguous()
         n1, n2, n3 = self.shape
-        out = torch.empty((n3, n1, n2), dtype=torch.float64, device=self.device)
+        out = torch.empty((n3, n2, n1), dtype=torch.float64, device=self.device)
         lib = nat.load()
         need = lib.qepw_inverse_real_workspace(self.handle)
         buf = ws.get(need)

@@ -13,7 +13,7 @@ reference module for the h_psi / Davidson path, so that
 Design (one GPU):
 
 * The Hamiltonian owns a device FFT plan whose rows are in "device order"
-  (plane waves grouped by z-stick), V(r) in plane-major (n3, n1, n2) layout,
+  (plane waves grouped by z-stick), V(r) in plane-major (n3, n2, n1) layout,
   the kinetic diagonal, and all projector terms packed into one device beta
   (npw, P) plus a block-diagonal D.  ``apply``/``apply_into`` accept host
   arrays in the reference's row order (or device tensors) and permute at the
@@ -194,10 +194,12 @@ class PlaneWaveHamiltonian:
         perm = self.plan.perm
         if real_potential is not None:
             rp = np.asarray(real_potential, dtype=np.float64)
-            if self.native_potential_layout:
-                plane_major = np.ascontiguousarray(rp.reshape(shape[2], shape[0], shape[1]))
+            # device layout (n3, n2, n1): plane z, row y, x fastest
+            if self.native_potential_layout:  # reference native layout (n3, n1, n2)
+                plane_major = np.ascontiguousarray(
+                    np.transpose(rp.reshape(shape[2], shape[0], shape[1]), (0, 2, 1)))
             else:
-                plane_major = np.ascontiguousarray(np.transpose(rp.reshape(shape), (2, 0, 1)))
+                plane_major = np.ascontiguousarray(np.transpose(rp.reshape(shape), (2, 1, 0)))
             self._potential64 = torch.from_numpy(plane_major).to(self.device)
             self._real_potential_host = rp
         else:
@@ -208,6 +210,9 @@ class PlaneWaveHamiltonian:
         beta, coupling = self._pack_projectors(perm)
         self._res = {torch.complex128: _PrecisionResources(self._potential64, kin, beta, coupling)}
         self.applications = 0
+        # set to a list to record (nb, dtype, start, stop) HIP events of every
+        # FFT-phase launch (used by the benchmark's live roofline)
+        self.fft_events = None
 
     # -- set-up -----------------------------------------------------------
     def _pack_projectors(self, perm):
@@ -265,9 +270,8 @@ class PlaneWaveHamiltonian:
     @property
     def real_potential(self) -> np.ndarray:
         if self._real_potential_host is None:
-            n1, n2, n3 = self.shape
-            plane = self._potential64.cpu().numpy()
-            self._real_potential_host = np.ascontiguousarray(np.transpose(plane, (1, 2, 0)))
+            plane = self._potential64.cpu().numpy()  # (n3, n2, n1)
+            self._real_potential_host = np.ascontiguousarray(np.transpose(plane, (2, 1, 0)))
         return self._real_potential_host
 
     @property
@@ -309,7 +313,16 @@ class PlaneWaveHamiltonian:
             return
         res = self._resources(src.dtype)
         with _measure(self.timers, "vloc_psi", self.device):
-            self.plan.apply_local(src, dst, res.potential, res.kinetic, self.workspace)
+            if self.fft_events is None:
+                self.plan.apply_local(src, dst, res.potential, res.kinetic, self.workspace)
+            else:
+                # HIP events on the launch stream around the FFT phase
+                start = torch.cuda.Event(enable_timing=True)
+                stop = torch.cuda.Event(enable_timing=True)
+                start.record()
+                self.plan.apply_local(src, dst, res.potential, res.kinetic, self.workspace)
+                stop.record()
+                self.fft_events.append((nb, src.dtype, start, stop))
         if res.beta is not None:
             p = res.nproj
             with _measure(self.timers, "h_psi:calbec", self.device):
@@ -475,9 +488,13 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
              residual_energy_scale=None, occupied_roots=None, initial_is_ritz=False,
              initial_is_orthonormal=False, initial_eigenvalues=None, initial_applied=None,
              mpi=None, global_dimension=None, global_row_indices=None, timers=None,
-             operator_into=None):
+             operator_into=None, device_output=False):
     """Lowest eigenpairs by restarted block Davidson (QE ``cegterg``) on the
-    GPU; semantics of ``qepy_pw/diagonalization.py:527-880``."""
+    GPU; semantics of ``qepy_pw/diagonalization.py:527-880``.
+
+    Extra keyword: ``device_output=True`` returns the eigenvectors as a
+    device tensor (npw, nroots) in reference row order instead of a host
+    array (no device-to-host copy of the vectors)."""
     diagonal = np.asarray(diagonal, dtype=float)
     mpi = mpi if mpi is not None else _SERIAL
     if getattr(mpi, "size", 1) > 1:
@@ -659,7 +676,7 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
         if number_unconverged == 0:
             if not track:
                 residual_norms = residual_norms_of(R)
-            return DavidsonResult(values, _result_vectors(X, perm_dev, npw), residual_norms,
+            return DavidsonResult(values, _result_vectors(X, perm_dev, npw, device_output), residual_norms,
                                   iteration, True, 0, happ)
         if active + number_unconverged > maximum_subspace:
             with _measure(timers, "cegterg:last", device):
@@ -672,11 +689,15 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
                 overlap[:active, :active] = np.eye(active, dtype=complex)
     if not track:
         residual_norms = residual_norms_of(R)
-    return DavidsonResult(values, _result_vectors(X, perm_dev, npw), residual_norms, completed,
+    return DavidsonResult(values, _result_vectors(X, perm_dev, npw, device_output), residual_norms, completed,
                           False, number_unconverged, happ)
 
 
-def _result_vectors(X, perm_dev, npw):
+def _result_vectors(X, perm_dev, npw, device_output=False):
+    if device_output:
+        ref = torch.empty((X.shape[0], npw), dtype=torch.complex128, device=X.device)
+        dv.permute_rows(1, perm_dev, X, ref, npw)
+        return ref.T
     if perm_dev is None:
         return np.asfortranarray(X.to(torch.complex128).cpu().numpy().T)
     return np.asfortranarray(_download_block(X, perm_dev, npw))

@@ -109,6 +109,41 @@ def test_cells_54_64_hpsi_parity():
         assert rel(device_ham(s).apply(x), oracle_ham(s).apply(x)) <= 1e-12, name
 
 
+def test_non_square_plane_generic_kernel():
+    """An orthorhombic cell (n1 != n2) runs the generic plane kernel."""
+    from qe_python_amd.basis import fft_shape, make_bases
+    from qe_python_amd.diagonalization import PlaneWaveHamiltonian
+    rng = np.random.default_rng(17)
+    reciprocal = 2.0 * np.pi * np.diag([1 / 9.0, 1 / 11.0, 1 / 7.5])
+    basis = make_bases(reciprocal, np.array([0.1, 0.2, 0.05]), 20.0)[0]
+    shape = fft_shape([basis], reciprocal, 80.0)
+    assert shape[0] != shape[1]
+    vr = rng.uniform(-1.0, 0.5, shape)
+    x = rng.standard_normal((len(basis), 5)) + 1j * rng.standard_normal((len(basis), 5))
+    ham = PlaneWaveHamiltonian(basis, np.zeros(shape, complex), (), real_potential=vr)
+    ref = OracleHamiltonian(basis.indices, basis.kinetic, shape, vr).apply(x)
+    assert rel(ham.apply(x), ref) <= 1e-12
+    ham32 = PlaneWaveHamiltonian(basis, np.zeros(shape, complex), (), real_potential=vr,
+                                 precision="fp32")
+    assert rel(ham32.apply(x), ref) <= 2e-6
+
+
+def test_team_kernel_sizes_against_oracle():
+    """Square planes of several team-FFT lengths (factorisations 4x6 ... 9x10)."""
+    from qe_python_amd.basis import make_bases
+    from qe_python_amd.diagonalization import PlaneWaveHamiltonian
+    rng = np.random.default_rng(23)
+    for n, n3, ecut in ((24, 24, 6.0), (30, 30, 10.0), (48, 45, 20.0), (90, 80, 60.0)):
+        reciprocal = 2.0 * np.pi / 12.0 * np.eye(3)
+        basis = make_bases(reciprocal, np.array([0.3, 0.1, 0.2]), ecut)[0]
+        shape = (n, n, n3)
+        vr = rng.uniform(-1.0, 0.5, shape)
+        x = rng.standard_normal((len(basis), 3)) + 1j * rng.standard_normal((len(basis), 3))
+        ham = PlaneWaveHamiltonian(basis, np.zeros(shape, complex), (), real_potential=vr)
+        ref = OracleHamiltonian(basis.indices, basis.kinetic, shape, vr).apply(x)
+        assert rel(ham.apply(x), ref) <= 1e-12, n
+
+
 def test_hermiticity_on_device(small):
     ham = device_ham(small)
     x, y = small.trial[:, :4], small.trial[:, 4:9]
