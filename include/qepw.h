@@ -194,6 +194,54 @@ int qepw_accumulate_projector_diagonal(const void* coupled, int64_t ldc, const v
                                        double multiplicity, double* diagonal,
                                        void* stream);
 
+/*
+ * out = scale * sum_b weights[b] |psi_b(r)|^2 with psi_b(r) the unnormalised
+ * backward FFT of vectors[:, b]; out: device double grid in (n3, n2, n1)
+ * layout; weights: device double (nb).  scale = 1/N gives dE/dV(r) of
+ * E = sum_b w_b <psi_b|H|psi_b>.  Deterministic (fixed-order band slices).
+ * Replaces: accumulate_density_serial / accumulate_density_serial_spatial
+ *   (qepy_pw/_native_fft.pyx:2537-2648) as called by
+ *   LocalPotentialWorkspace.accumulate_density (qepy_pw/basis.py:1210-1282).
+ */
+size_t qepw_accumulate_density_workspace(const qepw_plan* plan, int precision, int nb);
+int qepw_accumulate_density(const qepw_plan* plan, int precision, const void* vectors,
+                            int64_t ldv, int nb, const double* weights, double scale,
+                            double* out, void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * dst[g, b] = s[g * sstride] cfac[b] src[g, b]  (s, cfac: device double,
+ * either may be NULL for 1).  Forms the (k+G)_c psi blocks of the projector
+ * derivative of the nonlocal forces (qepy_pw/pw/scf.py:1580-1590, the
+ * ``1j * gk * modulated`` factor) and the occupation-weighted H psi of the
+ * band-energy gradient.
+ */
+int qepw_scale_rows(int precision, const void* src, int64_t lds, const double* s,
+                    int64_t sstride, const double* cfac, void* dst, int64_t ldd,
+                    int64_t nrows, int nb, void* stream);
+
+/*
+ * out[g, a*nch + i] = beta[g, i] exp(-i q_g . tau_a): the atom-packed
+ * projector matrix recomputed from positions (q: device double (npw, 3)
+ * Cartesian k+G, tau: device double (natom, 3) bohr).
+ * Replaces: phase_matrix (qepy_pw/_native_fft.pyx:1664-1699) and the packed
+ *   assembly of _nonlocal_projector_terms (qepy_pw/pw/scf.py:657-686).
+ */
+int qepw_build_projectors(int precision, const void* beta, int64_t ldb, int nch,
+                          const double* q, const double* tau, int natom, void* out,
+                          int64_t ldo, int64_t nrows, void* stream);
+
+/*
+ * out[a, c] = 2 sum_n f_n sum_{i in atom a} Re(conj(dO_c[i, n]) DO[i, n]):
+ * dE/dtau (the negative force) of the nonlocal energy, with dO_c = dO/dtau_c
+ * (three (P, nb) blocks, block stride dstride), DO = D O, atom a owning
+ * channels [off[a], off[a+1]) (device int32), f: device double (nb).
+ * Replaces: the force branch of _nonlocal_derivatives
+ *   (qepy_pw/pw/scf.py:1591-1599).
+ */
+int qepw_projector_forces(int precision, const void* dO, int64_t lddo, int64_t dstride,
+                          const void* DO, int64_t lddO, const double* f, int nb,
+                          const int32_t* off, int natom, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,15 @@ SIGNATURES = [
                                   c_int64, c_int64, c_int, c_void_p]),
     ("qepw_accumulate_projector_diagonal", c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int64,
                                                    c_int, c_double, c_void_p, c_void_p]),
+    ("qepw_accumulate_density_workspace", c_size_t, [c_void_p, c_int, c_int]),
+    ("qepw_accumulate_density", c_int, [c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p,
+                                        c_double, c_void_p, c_void_p, c_size_t, c_void_p]),
+    ("qepw_scale_rows", c_int, [c_int, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p,
+                                c_int64, c_int64, c_int, c_void_p]),
+    ("qepw_build_projectors", c_int, [c_int, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_int,
+                                      c_void_p, c_int64, c_int64, c_void_p]),
+    ("qepw_projector_forces", c_int, [c_int, c_void_p, c_int64, c_int64, c_void_p, c_int64,
+                                      c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
 ]
 
 _LIB: ctypes.CDLL | None = None

@@ -408,3 +408,55 @@ int qepw_accumulate_projector_diagonal(const void* coupled, int64_t ldc, const v
 }
 
 }  // extern "C"
+
+extern "C" {
+
+size_t qepw_accumulate_density_workspace(const qepw_plan* plan, int precision, int nb) {
+  if (!plan || (precision != QEPW_C128 && precision != QEPW_C64)) return 0;
+  return qepw::density_workspace_bytes(plan->p, precision, nb);
+}
+
+int qepw_accumulate_density(const qepw_plan* plan, int precision, const void* vectors,
+                            int64_t ldv, int nb, const double* weights, double scale,
+                            double* out, void* workspace, size_t ws_bytes, void* stream) {
+  if (!plan || !out || !workspace) return fail(QEPW_ERR_ARG, "qepw_accumulate_density: NULL argument");
+  if (nb < 0) return fail(QEPW_ERR_ARG, "qepw_accumulate_density: negative band count");
+  if (nb > 0 && (!vectors || !weights || ldv < plan->p.npw))
+    return fail(QEPW_ERR_ARG, "qepw_accumulate_density: bad vectors / weights");
+  return finish(qepw::accumulate_density(plan->p, precision, vectors, ldv, nb, weights, scale, out,
+                                         workspace, ws_bytes, static_cast<hipStream_t>(stream)),
+                "qepw_accumulate_density");
+}
+
+int qepw_scale_rows(int precision, const void* src, int64_t lds, const double* s,
+                    int64_t sstride, const double* cfac, void* dst, int64_t ldd, int64_t nrows,
+                    int nb, void* stream) {
+  if (nb > 0 && nrows > 0 && (!src || !dst || lds < nrows || ldd < nrows || sstride < 0))
+    return fail(QEPW_ERR_ARG, "qepw_scale_rows: bad argument");
+  return finish(qepw::scale_rows(precision, src, lds, s, sstride, cfac, dst, ldd, nrows, nb,
+                                 static_cast<hipStream_t>(stream)),
+                "qepw_scale_rows");
+}
+
+int qepw_build_projectors(int precision, const void* beta, int64_t ldb, int nch, const double* q,
+                          const double* tau, int natom, void* out, int64_t ldo, int64_t nrows,
+                          void* stream) {
+  if (natom > 0 && nch > 0 && nrows > 0 &&
+      (!beta || !q || !tau || !out || ldb < nrows || ldo < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_build_projectors: bad argument");
+  return finish(qepw::build_projectors(precision, beta, ldb, nch, q, tau, natom, out, ldo, nrows,
+                                       static_cast<hipStream_t>(stream)),
+                "qepw_build_projectors");
+}
+
+int qepw_projector_forces(int precision, const void* dO, int64_t lddo, int64_t dstride,
+                          const void* DO, int64_t lddO, const double* f, int nb,
+                          const int32_t* off, int natom, double* out, void* stream) {
+  if (natom > 0 && (!dO || !DO || !f || !off || !out || nb < 0))
+    return fail(QEPW_ERR_ARG, "qepw_projector_forces: bad argument");
+  return finish(qepw::projector_forces(precision, dO, lddo, dstride, DO, lddO, f, nb, off, natom,
+                                       out, static_cast<hipStream_t>(stream)),
+                "qepw_projector_forces");
+}
+
+}  // extern "C"

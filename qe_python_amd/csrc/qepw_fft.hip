@@ -369,6 +369,228 @@ int apply_local(const Plan& p, int precision, const void* vectors, int64_t ldv, 
   return QEPW_ERR_ARG;
 }
 
+// ---- density accumulation -------------------------------------------------
+// rho(r) = scale * sum_b w_b |psi_b(r)|^2 with psi_b(r) the unnormalised
+// backward FFT (reference: accumulate_density_serial, _native_fft.pyx:
+// 2592-2648).  The backward z stage is k_sticks_inverse; the plane stage
+// loops over a fixed slice of bands inside each workgroup and accumulates in
+// LDS / registers, writing one partial plane per (slice, z); a last kernel
+// sums the slices in a fixed order.  No atomics: results are reproducible.
+
+template <typename T, int N>
+__global__ __launch_bounds__(kTeamThreads) void k_density_team(
+    const cplx<T>* __restrict__ work, int64_t band_stride, int nst,
+    const int* __restrict__ stick_q, const int* __restrict__ colmap, int nxc, int nb,
+    int bands_per_slice, const double* __restrict__ weights, double* __restrict__ partial,
+    int accumulate, const cplx<T>* __restrict__ tw) {
+  using G = TeamGeom<N>;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* Q = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* scratch_all = Q + nxc * N;
+  double* D = reinterpret_cast<double*>(scratch_all + G::teams * N);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int team_in_wave = lane / G::M;
+  const int t = lane - team_in_wave * G::M;
+  const bool in_team = team_in_wave < G::per_wave;
+  const int team = wave * G::per_wave + team_in_wave;
+  cplx<T>* S = scratch_all + team * N;
+  const int z = blockIdx.x;
+  const int slice = blockIdx.y;
+  const int b0 = slice * bands_per_slice;
+  const int b1 = min(nb, b0 + bands_per_slice);
+  const int nq = nxc * N;
+  const cplx<T> zero = mk<T>(T(0), T(0));
+  for (int i = tid; i < N * N; i += kTeamThreads) D[i] = 0.0;
+  for (int b = b0; b < b1; ++b) {
+    const cplx<T>* w = work + (int64_t)b * band_stride + (int64_t)z * nst;
+    const double wb = weights[b];
+    for (int i = tid; i < nq; i += kTeamThreads) Q[i] = zero;
+    __syncthreads();
+    for (int s = tid; s < nst; s += kTeamThreads) Q[stick_q[s]] = w[s];
+    __syncthreads();
+    if (in_team) {
+      for (int c = team; c < nxc; c += G::teams) {
+        cplx<T>* col = Q + c * N;
+        team_fft<T, N, +1>(t, S, tw, [&](int j) { return col[j]; },
+                           [&](int k, cplx<T> v) { col[k] = v; });
+      }
+    }
+    __syncthreads();
+    if (in_team) {
+      for (int y = team; y < N; y += G::teams) {
+        double* drow = D + y * N;
+        team_fft<T, N, +1>(
+            t, S, tw,
+            [&](int j) {
+              const int c = colmap[j];
+              return c >= 0 ? Q[c * N + y] : zero;
+            },
+            [&](int k, cplx<T> v) {
+              drow[k] += wb * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+            });
+      }
+    }
+    __syncthreads();
+  }
+  double* out = partial + ((int64_t)slice * gridDim.x + z) * N * N;
+  for (int i = tid; i < N * N; i += kTeamThreads) out[i] = (accumulate ? out[i] : 0.0) + D[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kPlaneThreads) void k_density_plane(
+    const cplx<T>* __restrict__ work, int64_t band_stride, int nst,
+    const int* __restrict__ plane_of, const int* __restrict__ xcols, int nxcols, int nb,
+    int bands_per_slice, const double* __restrict__ weights, double* __restrict__ partial,
+    int accumulate, int n1, int n2, LinePlan p1, LinePlan p2,
+    const cplx<T>* __restrict__ tw1, const cplx<T>* __restrict__ tw2) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* P = reinterpret_cast<cplx<T>*>(smem_raw);
+  constexpr int kPer = kPlanePoints;  // plane points owned per thread (np <= NT * 15)
+  const int tid = threadIdx.x;
+  const int z = blockIdx.x;
+  const int slice = blockIdx.y;
+  const int b0 = slice * bands_per_slice;
+  const int b1 = min(nb, b0 + bands_per_slice);
+  const int np = n1 * n2;
+  const cplx<T> zero = mk<T>(T(0), T(0));
+  double acc[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) acc[k] = 0.0;
+  for (int b = b0; b < b1; ++b) {
+    const cplx<T>* w = work + (int64_t)b * band_stride + (int64_t)z * nst;
+    const double wb = weights[b];
+    for (int i = tid; i < np; i += kPlaneThreads) P[i] = zero;
+    __syncthreads();
+    for (int s = tid; s < nst; s += kPlaneThreads) P[plane_of[s]] = w[s];
+    __syncthreads();
+    lds_fft<T, kPlaneThreads, kPlanePoints>(P, nxcols, n2, 1, xcols, p2, tw2, +1, tid);
+    lds_fft<T, kPlaneThreads, kPlanePoints>(P, n2, 1, n2, nullptr, p1, tw1, +1, tid);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = tid + k * kPlaneThreads;
+      if (i < np) {
+        const cplx<T> v = P[i];
+        acc[k] += wb * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+      }
+    }
+    __syncthreads();
+  }
+  double* out = partial + ((int64_t)slice * gridDim.x + z) * np;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = tid + k * kPlaneThreads;  // LDS plane index x * n2 + y
+    if (i < np) {
+      const int x = i / n2, y = i - x * n2;
+      const int o = y * n1 + x;
+      out[o] = (accumulate ? out[o] : 0.0) + acc[k];
+    }
+  }
+}
+
+__global__ void k_sum_slices(const double* __restrict__ partial, int slices, int64_t n,
+                             double scale, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < slices; ++k) s += partial[(int64_t)k * n + i];
+  out[i] = scale * s;
+}
+
+constexpr int kDensitySlices = 16;
+
+size_t density_workspace_bytes(const Plan& p, int precision, int nb) {
+  const size_t ngrid = (size_t)p.n1 * p.n2 * p.n3;
+  return apply_local_band_bytes(p, precision) * (size_t)(nb > 0 ? nb : 1) +
+         (size_t)kDensitySlices * ngrid * sizeof(double) + 256;
+}
+
+template <typename T>
+static size_t density_team_lds(int n, int nxc) {
+  const size_t base = team_lds_bytes<T>(n, nxc);
+  return base == 0 ? 0 : base + (size_t)n * n * sizeof(double);
+}
+
+template <typename T>
+int accumulate_density_t(const Plan& p, const void* vectors, int64_t ldv, int nb,
+                         const double* weights, double scale, double* out, void* workspace,
+                         size_t ws_bytes, hipStream_t stream) {
+  const StickTables& st = p.sphere;
+  const size_t ngrid = (size_t)p.n1 * p.n2 * p.n3;
+  const size_t np = (size_t)p.n1 * p.n2;
+  if (st.group * p.n3 > lds_fft_capacity(p.line[2], kStickThreads, kStickPts))
+    return QEPW_ERR_UNSUPPORTED;
+  const size_t team_lds = (p.n1 == p.n2 && st.d_stickq) ? density_team_lds<T>(p.n1, st.nxcols) : 0;
+  const bool team = team_lds > 0 && team_lds <= (size_t)kMaxLdsBytes;
+  if (!team) {
+    int rc = check_fft_limits<T>(p, st);
+    if (rc) return rc;
+  }
+  const size_t partial_bytes = (size_t)kDensitySlices * ngrid * sizeof(double);
+  const size_t band_bytes = (size_t)p.n3 * st.nst * sizeof(cplx<T>);
+  if (ws_bytes < partial_bytes + band_bytes) return QEPW_ERR_WORKSPACE;
+  double* partial = static_cast<double*>(workspace);
+  cplx<T>* work = reinterpret_cast<cplx<T>*>(static_cast<char*>(workspace) + partial_bytes);
+  int chunk = (int)((ws_bytes - partial_bytes) / band_bytes);
+  if (chunk > 65535) chunk = 65535;
+  const size_t stick_lds = (size_t)st.group * p.n3 * sizeof(cplx<T>);
+  const size_t plane_lds = np * sizeof(cplx<T>);
+  set_lds_attrs<T>(stick_lds, 0);
+  if (!team)
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density_plane<T>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_lds);
+  const int64_t bstride = (int64_t)p.n3 * st.nst;
+  if (nb <= 0) {
+    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((ngrid + 255) / 256)), dim3(256), 0,
+                       stream, partial, 0, (int64_t)ngrid, scale, out);
+    return hipGetLastError() == hipSuccess ? QEPW_OK : QEPW_ERR_HIP;
+  }
+  for (int b0 = 0; b0 < nb; b0 += chunk) {
+    const int nbc = (nb - b0) < chunk ? (nb - b0) : chunk;
+    const cplx<T>* src = static_cast<const cplx<T>*>(vectors) + (int64_t)b0 * ldv;
+    hipLaunchKernelGGL(k_sticks_inverse<T>, dim3(st.ngroups, nbc), dim3(kStickThreads),
+                       stick_lds, stream, src, ldv, work, bstride, st.nst, p.n3, st.d_off,
+                       st.d_gpos, st.group, p.line[2], twiddles<T>(p, 2));
+    const int per = (nbc + kDensitySlices - 1) / kDensitySlices;
+    const dim3 grid(p.n3, kDensitySlices);
+    const int acc = b0 > 0 ? 1 : 0;
+    bool launched = false;
+    if (team) {
+#define QEPW_LAUNCH_DENSITY(NN)                                                              \
+  if (!launched && p.n1 == NN) {                                                             \
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density_team<T, NN>),              \
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)team_lds);          \
+    hipLaunchKernelGGL((k_density_team<T, NN>), grid, dim3(kTeamThreads), team_lds, stream, \
+                       work, bstride, st.nst, st.d_stickq, st.d_colmap, st.nxcols, nbc, per, \
+                       weights + b0, partial, acc, twiddles<T>(p, 0));                       \
+    launched = true;                                                                         \
+  }
+      QEPW_TEAM_SIZES(QEPW_LAUNCH_DENSITY)
+#undef QEPW_LAUNCH_DENSITY
+    }
+    if (!launched)
+      hipLaunchKernelGGL(k_density_plane<T>, grid, dim3(kPlaneThreads), plane_lds, stream, work,
+                         bstride, st.nst, st.d_plane, st.d_xcols, st.nxcols, nbc, per,
+                         weights + b0, partial, acc, p.n1, p.n2, p.line[0], p.line[1],
+                         twiddles<T>(p, 0), twiddles<T>(p, 1));
+  }
+  hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((ngrid + 255) / 256)), dim3(256), 0, stream,
+                     partial, kDensitySlices, (int64_t)ngrid, scale, out);
+  return hipGetLastError() == hipSuccess ? QEPW_OK : QEPW_ERR_HIP;
+}
+
+int accumulate_density(const Plan& p, int precision, const void* vectors, int64_t ldv, int nb,
+                       const double* weights, double scale, double* out, void* workspace,
+                       size_t ws_bytes, hipStream_t stream) {
+  if (precision == QEPW_C128)
+    return accumulate_density_t<double>(p, vectors, ldv, nb, weights, scale, out, workspace,
+                                        ws_bytes, stream);
+  if (precision == QEPW_C64)
+    return accumulate_density_t<float>(p, vectors, ldv, nb, weights, scale, out, workspace,
+                                       ws_bytes, stream);
+  return QEPW_ERR_ARG;
+}
+
 size_t inverse_real_bytes(const Plan& p) {
   return (size_t)p.n1 * p.n2 * p.n3 * sizeof(double2);
 }

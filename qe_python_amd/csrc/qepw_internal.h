@@ -15,6 +15,10 @@ size_t apply_local_band_bytes(const Plan& p, int precision);
 int apply_local(const Plan& p, int precision, const void* vectors, int64_t ldv, int nb,
                 const void* potential, const void* kinetic, void* result, int64_t ldr,
                 void* workspace, size_t ws_bytes, hipStream_t stream);
+size_t density_workspace_bytes(const Plan& p, int precision, int nb);
+int accumulate_density(const Plan& p, int precision, const void* vectors, int64_t ldv, int nb,
+                       const double* weights, double scale, double* out, void* workspace,
+                       size_t ws_bytes, hipStream_t stream);
 size_t inverse_real_bytes(const Plan& p);
 int inverse_real(const Plan& p, const void* grid_g, void* out_real, void* workspace,
                  size_t ws_bytes, hipStream_t stream);
@@ -46,5 +50,14 @@ int permute_rows(int src_precision, int dst_precision, int direction, const int*
                  hipStream_t stream);
 int row_real_dot(const void* a, int64_t lda, const void* b, int64_t ldb, int64_t nrows,
                  int ncols, double scale, double* acc, hipStream_t stream);
+int scale_rows(int precision, const void* src, int64_t lds, const double* s, int64_t sstride,
+               const double* cfac, void* dst, int64_t ldd, int64_t nrows, int nb,
+               hipStream_t stream);
+int build_projectors(int precision, const void* beta, int64_t ldb, int nch, const double* q,
+                     const double* tau, int natom, void* out, int64_t ldo, int64_t nrows,
+                     hipStream_t stream);
+int projector_forces(int precision, const void* dO, int64_t lddo, int64_t dstride,
+                     const void* DO, int64_t lddO, const double* f, int nb, const int* off,
+                     int natom, double* out, hipStream_t stream);
 
 }  // namespace qepw

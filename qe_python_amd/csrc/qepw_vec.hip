@@ -154,6 +154,68 @@ __global__ void k_row_real_dot(const double2* __restrict__ a, int64_t lda,
   acc[g] += scale * s;
 }
 
+// dst[g, b] = s[g] c[b] src[g, b] (either factor may be absent): the
+// (k+G)_c psi factor of the projector derivative (pw/scf.py:1580-1590) and
+// the occupation-weighted H psi of the band-energy gradient
+template <typename T>
+__global__ void k_scale_rows(const cplx<T>* __restrict__ src, int64_t lds,
+                             const double* __restrict__ s, int64_t sstride,
+                             const double* __restrict__ cfac,
+                             cplx<T>* __restrict__ dst, int64_t ldd, int64_t nrows) {
+  const int64_t b = blockIdx.y;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrows) return;
+  const T f = (T)((s ? s[g * sstride] : 1.0) * (cfac ? cfac[b] : 1.0));
+  const cplx<T> v = src[b * lds + g];
+  dst[b * ldd + g] = mk<T>(f * v.x, f * v.y);
+}
+
+// B[g, a*nch + i] = beta[g, i] * exp(-i q_g . tau_a): the atom-packed
+// projector of pw/scf.py:657-686 recomputed from positions on the device.
+template <typename T>
+__global__ void k_build_projectors(const cplx<T>* __restrict__ beta, int64_t ldb, int nch,
+                                   const double* __restrict__ q, const double* __restrict__ tau,
+                                   int natom, cplx<T>* __restrict__ out, int64_t ldo,
+                                   int64_t nrows) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y;
+  if (g >= nrows || a >= natom) return;
+  const double arg = -(q[3 * g] * tau[3 * a] + q[3 * g + 1] * tau[3 * a + 1] +
+                       q[3 * g + 2] * tau[3 * a + 2]);
+  double sn, cs;
+  sincos(arg, &sn, &cs);
+  for (int i = 0; i < nch; ++i) {
+    const cplx<T> bv = beta[(int64_t)i * ldb + g];
+    const double re = (double)bv.x * cs - (double)bv.y * sn;
+    const double im = (double)bv.x * sn + (double)bv.y * cs;
+    out[(int64_t)(a * nch + i) * ldo + g] = mk<T>((T)re, (T)im);
+  }
+}
+
+// out[a*3 + c] = 2 sum_n f_n sum_{i in atom a} Re(conj(dO_c[i, n]) DO[i, n]),
+// the force-branch contraction of pw/scf.py:1591-1599 (dE/dtau = -F).
+// dO: three (P, nb) blocks with column stride lddo and block stride
+// dstride; DO = D O; atom a owns channels [off[a], off[a+1]).
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_projector_forces(
+    const cplx<T>* __restrict__ dO, int64_t lddo, int64_t dstride,
+    const cplx<T>* __restrict__ DO, int64_t lddO, const double* __restrict__ f, int nb,
+    const int* __restrict__ off, double* __restrict__ out) {
+  __shared__ double sh[kRedThreads];
+  const int a = blockIdx.x, c = blockIdx.y;
+  const int i0 = off[a], i1 = off[a + 1];
+  const int nch = i1 - i0;
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < nch * nb; e += kRedThreads) {
+    const int n = e / nch, i = i0 + (e - n * nch);
+    const cplx<T> d = dO[c * dstride + (int64_t)n * lddo + i];
+    const cplx<T> o = DO[(int64_t)n * lddO + i];
+    acc += f[n] * ((double)d.x * o.x + (double)d.y * o.y);
+  }
+  const double s = block_sum<T>(acc, sh);
+  if (threadIdx.x == 0) out[a * 3 + c] = 2.0 * s;
+}
+
 static inline dim3 rows_grid(int64_t nrows, int nb) {
   return dim3((unsigned)((nrows + 255) / 256), (unsigned)nb);
 }
@@ -282,6 +344,54 @@ int row_real_dot(const void* a, int64_t lda, const void* b, int64_t ldb, int64_t
   hipLaunchKernelGGL(k_row_real_dot, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0,
                      stream, static_cast<const double2*>(a), lda,
                      static_cast<const double2*>(b), ldb, nrows, ncols, scale, acc);
+  return status();
+}
+
+}  // namespace qepw
+
+namespace qepw {
+
+int scale_rows(int precision, const void* src, int64_t lds, const double* s, int64_t sstride,
+               const double* cfac, void* dst, int64_t ldd, int64_t nrows, int nb,
+               hipStream_t stream) {
+  if (nb <= 0 || nrows <= 0) return QEPW_OK;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_scale_rows<double>, rows_grid(nrows, nb), dim3(256), 0, stream,
+                         static_cast<const double2*>(src), lds, s, sstride, cfac,
+                         static_cast<double2*>(dst), ldd, nrows),
+      hipLaunchKernelGGL(k_scale_rows<float>, rows_grid(nrows, nb), dim3(256), 0, stream,
+                         static_cast<const float2*>(src), lds, s, sstride, cfac,
+                         static_cast<float2*>(dst), ldd, nrows));
+  return status();
+}
+
+int build_projectors(int precision, const void* beta, int64_t ldb, int nch, const double* q,
+                     const double* tau, int natom, void* out, int64_t ldo, int64_t nrows,
+                     hipStream_t stream) {
+  if (natom <= 0 || nrows <= 0 || nch <= 0) return QEPW_OK;
+  if (natom > 65535) return QEPW_ERR_UNSUPPORTED;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_build_projectors<double>, rows_grid(nrows, natom), dim3(256), 0,
+                         stream, static_cast<const double2*>(beta), ldb, nch, q, tau, natom,
+                         static_cast<double2*>(out), ldo, nrows),
+      hipLaunchKernelGGL(k_build_projectors<float>, rows_grid(nrows, natom), dim3(256), 0,
+                         stream, static_cast<const float2*>(beta), ldb, nch, q, tau, natom,
+                         static_cast<float2*>(out), ldo, nrows));
+  return status();
+}
+
+int projector_forces(int precision, const void* dO, int64_t lddo, int64_t dstride,
+                     const void* DO, int64_t lddO, const double* f, int nb, const int* off,
+                     int natom, double* out, hipStream_t stream) {
+  if (natom <= 0) return QEPW_OK;
+  if (natom > 0x7fffffff / 3) return QEPW_ERR_UNSUPPORTED;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_projector_forces<double>, dim3(natom, 3), dim3(kRedThreads), 0,
+                         stream, static_cast<const double2*>(dO), lddo, dstride,
+                         static_cast<const double2*>(DO), lddO, f, nb, off, out),
+      hipLaunchKernelGGL(k_projector_forces<float>, dim3(natom, 3), dim3(kRedThreads), 0,
+                         stream, static_cast<const float2*>(dO), lddo, dstride,
+                         static_cast<const float2*>(DO), lddO, f, nb, off, out));
   return status();
 }
 

@@ -1,0 +1,138 @@
+"""GPU parity of the band-energy backward (``qe_python_amd.gradients``).
+
+Bars: FP64 energy within 1e-12 relative of the oracle; FP64 gradients
+(dE/dV, dE/dtau, dE/dpsi*) within 1e-10 relative (Frobenius) of the oracle
+gradients and within 1e-6 of central finite differences of the device
+energy (h = 1e-5 bohr for tau, exact linear / quadratic steps for V and psi);
+FP32 (complex64) energy within 2e-6 and gradients within 2e-5 relative.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle.forces import band_energy_gradients
+from qe_python_amd.synthetic import make_system
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def setup(name, nb, seed=3):
+    from qe_python_amd.gradients import BandEnergyModel
+    s = make_system(name)
+    rng = np.random.default_rng(seed)
+    psi = s.trial[:, :nb] * 2.0
+    f = rng.uniform(0.5, 2.0, nb)
+    model = BandEnergyModel(s.basis, s.shape, s.projectors.beta_species, s.projectors.couplings)
+    ref = band_energy_gradients(s.basis.indices, s.basis.kinetic, s.basis.vectors, s.shape,
+                                s.real_potential, s.projectors.beta_species,
+                                s.projectors.couplings, s.projectors.positions, psi, f)
+    return s, psi, f, model, ref
+
+
+def run(model, s, psi, f, precision="fp64", positions=None, potential=None):
+    from qe_python_amd.gradients import band_energy
+    p = torch.tensor(psi, device=DEV, requires_grad=True)
+    v = torch.tensor(s.real_potential if potential is None else potential, device=DEV,
+                     requires_grad=True)
+    t = torch.tensor(s.projectors.positions if positions is None else positions, device=DEV,
+                     requires_grad=True)
+    e = band_energy(model, p, v, t, f, precision=precision)
+    e.backward()
+    return (float(e.detach()), v.grad.cpu().numpy(), t.grad.cpu().numpy(), p.grad.cpu().numpy())
+
+
+@pytest.fixture(scope="module")
+def small():
+    return setup("small", 8)
+
+
+def test_small_fp64_against_oracle(small):
+    s, psi, f, model, (e, dv, dtau, gpsi) = small
+    e_d, dv_d, dtau_d, gpsi_d = run(model, s, psi, f)
+    assert e_d == pytest.approx(e, rel=1e-12)
+    assert rel(dv_d, dv) < 1e-10
+    assert rel(dtau_d, dtau) < 1e-10
+    assert rel(gpsi_d, gpsi) < 1e-10
+
+
+def test_small_fp32_against_oracle(small):
+    s, psi, f, model, (e, dv, dtau, gpsi) = small
+    e_d, dv_d, dtau_d, gpsi_d = run(model, s, psi, f, precision="fp32")
+    assert e_d == pytest.approx(e, rel=2e-6)
+    assert rel(dv_d, dv) < 2e-5
+    assert rel(dtau_d, dtau) < 2e-5
+    assert rel(gpsi_d, gpsi) < 2e-5
+
+
+def test_small_device_gradients_against_central_differences(small):
+    from qe_python_amd.gradients import band_energy
+    s, psi, f, model, _ = small
+    _, dv_d, dtau_d, gpsi_d = run(model, s, psi, f)
+
+    def energy(positions=None, potential=None, p=None):
+        with torch.no_grad():
+            return float(band_energy(
+                model, torch.tensor(psi if p is None else p, device=DEV),
+                torch.tensor(s.real_potential if potential is None else potential, device=DEV),
+                torch.tensor(s.projectors.positions if positions is None else positions,
+                             device=DEV), f))
+
+    h = 1e-4  # bohr; E is a smooth function of tau (truncation ~h^2)
+    for a, c in [(0, 0), (1, 2)]:
+        plus = s.projectors.positions.copy()
+        minus = s.projectors.positions.copy()
+        plus[a, c] += h
+        minus[a, c] -= h
+        fd = (energy(positions=plus) - energy(positions=minus)) / (2 * h)
+        assert fd == pytest.approx(dtau_d[a, c], rel=1e-5, abs=1e-8)
+    hv = 0.1  # E is linear in V: the central difference is exact
+    for idx in [(0, 0, 0), (7, 20, 33)]:
+        vp, vm = s.real_potential.copy(), s.real_potential.copy()
+        vp[idx] += hv
+        vm[idx] -= hv
+        fd = (energy(potential=vp) - energy(potential=vm)) / (2 * hv)
+        assert fd == pytest.approx(dv_d[idx], rel=1e-8, abs=1e-12)
+    d = np.random.default_rng(9).standard_normal(psi.shape) * (1 + 1j)
+    hp = 1e-2  # E is quadratic in psi: the central difference is exact
+    fd = (energy(p=psi + hp * d) - energy(p=psi - hp * d)) / (2 * hp)
+    assert fd == pytest.approx(float(np.real(np.vdot(gpsi_d, d))), rel=1e-6)
+
+
+def test_medium_workload_backward_against_oracle():
+    """The differentiation workload: 72^3 grid, 64 atoms, 128 occupied bands
+    (f = 2), complex128 and complex64."""
+    from qe_python_amd.gradients import BandEnergyModel
+    s = make_system("medium")
+    nb = 128
+    psi = s.trial[:, :nb]
+    f = np.full(nb, 2.0)
+    model = BandEnergyModel(s.basis, s.shape, s.projectors.beta_species, s.projectors.couplings)
+    e, dv, dtau, gpsi = band_energy_gradients(
+        s.basis.indices, s.basis.kinetic, s.basis.vectors, s.shape, s.real_potential,
+        s.projectors.beta_species, s.projectors.couplings, s.projectors.positions, psi, f)
+    e_d, dv_d, dtau_d, gpsi_d = run(model, s, psi, f)
+    assert e_d == pytest.approx(e, rel=1e-12)
+    assert rel(dv_d, dv) < 1e-10 and rel(dtau_d, dtau) < 1e-10 and rel(gpsi_d, gpsi) < 1e-10
+    e32, dv32, dtau32, gpsi32 = run(model, s, psi, f, precision="fp32")
+    assert e32 == pytest.approx(e, rel=2e-6)
+    assert rel(dv32, dv) < 2e-5 and rel(dtau32, dtau) < 2e-5 and rel(gpsi32, gpsi) < 2e-5
+
+
+def test_argument_errors(small):
+    from qe_python_amd.gradients import band_energy
+    s, psi, f, model, _ = small
+    p = torch.tensor(psi, device=DEV)
+    v = torch.tensor(s.real_potential, device=DEV)
+    t = torch.tensor(s.projectors.positions, device=DEV)
+    with pytest.raises(ValueError):
+        band_energy(model, p[:-1], v, t, f)
+    with pytest.raises(ValueError):
+        band_energy(model, p, v[:-1], t, f)
+    with pytest.raises(ValueError):
+        band_energy(model, p, v, t, f[:-1])
