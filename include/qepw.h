@@ -35,8 +35,9 @@ extern "C" {
 #define QEPW_C128 0 /* complex128 storage and FP64 arithmetic */
 #define QEPW_C64 1  /* complex64 storage and FP32 arithmetic  */
 
-#define QEPW_OP_N 0 /* op(A) = A   */
-#define QEPW_OP_C 1 /* op(A) = A^H */
+#define QEPW_OP_N 0  /* op(A) = A, op(B) = B        */
+#define QEPW_OP_C 1  /* op(A) = A^H                 */
+#define QEPW_OP_BC 2 /* op(B) = B^H (or-ed with op) */
 
 #define QEPW_OK 0
 #define QEPW_ERR_ARG (-1)         /* invalid argument (ValueError)            */
@@ -102,9 +103,10 @@ int qepw_inverse_real(const qepw_plan* plan, const void* grid_g, double* out,
                       void* workspace, size_t ws_bytes, void* stream);
 
 /*
- * C (m x n) = alpha op(A) B + beta C, complex, column-major, on MFMA
+ * C (m x n) = alpha op(A) op(B) + beta C, complex, column-major, on MFMA
  * (v_mfma_f64_16x16x4f64 for QEPW_C128, v_mfma_f32_16x16x4f32 for QEPW_C64).
- * op_a = QEPW_OP_N: A is m x k;  QEPW_OP_C: A is k x m and op(A) = A^H.
+ * op_a = QEPW_OP_N: A is m x k;  QEPW_OP_C: A is k x m and op(A) = A^H;
+ * or-ed with QEPW_OP_BC: B is n x k and op(B) = B^H (else B is k x n).
  * alpha, beta: host (re, im) pairs.  A^H B with large k is split over k with
  * a fixed-order (deterministic) reduction through ``workspace``.
  * Replaces: the zgemm bridge qepy_zgemm64 (qepy_pw/_native_fft.pyx:595-634)
@@ -241,6 +243,29 @@ int qepw_build_projectors(int precision, const void* beta, int64_t ldb, int nch,
 int qepw_projector_forces(int precision, const void* dO, int64_t lddo, int64_t dstride,
                           const void* DO, int64_t lddO, const double* f, int nb,
                           const int32_t* off, int natom, double* out, void* stream);
+
+/*
+ * Pieces of the device Rayleigh-Ritz that replaces the host zhegvd call of
+ * davidson (generalized_eigh, qepy_pw/_native_fft.pyx:667-691, via
+ * _lowest_generalized_eigh, qepy_pw/diagonalization.py:83-110).  All operate
+ * on column-major complex128 matrices in device memory.
+ *
+ * qepw_hermitian_complete: rows [first, active) of A were just written for
+ *   columns [0, active); mirror them into the upper block and symmetrise the
+ *   new diagonal block (update_projected, diagonalization.py:658-697).
+ * qepw_shift_scale: bound[0] = max_i sum_j |A_ij - sigma delta_ij| (device
+ *   double, an upper bound of ||A - sigma I||_2) and X = (A - sigma I)/bound.
+ * qepw_hermitize: X = (X + X^H)/2 in place.
+ * qepw_sign_check: out[0] = ||X2 - I||_F^2, out[1] = Re trace(X) (device).
+ * qepw_set_diagonal: A[:n, :n] = diag(d), identity when d is NULL.
+ */
+int qepw_hermitian_complete(void* a, int64_t lda, int first, int active, void* stream);
+int qepw_shift_scale(const void* a, int64_t lda, int n, double sigma, void* x, int64_t ldx,
+                     double* bound, void* stream);
+int qepw_hermitize(void* x, int64_t ldx, int n, void* stream);
+int qepw_sign_check(const void* x, int64_t ldx, const void* x2, int64_t ld2, int n,
+                    double* out, void* stream);
+int qepw_set_diagonal(void* a, int64_t lda, int n, const double* d, void* stream);
 
 #ifdef __cplusplus
 }

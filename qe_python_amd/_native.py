@@ -19,6 +19,7 @@ C128 = 0
 C64 = 1
 OP_N = 0
 OP_C = 1
+OP_BC = 2
 
 OK = 0
 ERR_ARG = -1
@@ -69,6 +70,12 @@ SIGNATURES = [
                                       c_void_p, c_int64, c_int64, c_void_p]),
     ("qepw_projector_forces", c_int, [c_int, c_void_p, c_int64, c_int64, c_void_p, c_int64,
                                       c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
+    ("qepw_hermitian_complete", c_int, [c_void_p, c_int64, c_int, c_int, c_void_p]),
+    ("qepw_shift_scale", c_int, [c_void_p, c_int64, c_int, c_double, c_void_p, c_int64, c_void_p,
+                                 c_void_p]),
+    ("qepw_hermitize", c_int, [c_void_p, c_int64, c_int, c_void_p]),
+    ("qepw_sign_check", c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int, c_void_p, c_void_p]),
+    ("qepw_set_diagonal", c_int, [c_void_p, c_int64, c_int, c_void_p, c_void_p]),
 ]
 
 _LIB: ctypes.CDLL | None = None

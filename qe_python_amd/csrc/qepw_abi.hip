@@ -460,3 +460,43 @@ int qepw_projector_forces(int precision, const void* dO, int64_t lddo, int64_t d
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int qepw_hermitian_complete(void* a, int64_t lda, int first, int active, void* stream) {
+  if (!a || first < 0 || active < first || lda < active)
+    return fail(QEPW_ERR_ARG, "qepw_hermitian_complete: bad argument");
+  return finish(qepw::hermitian_complete(a, lda, first, active, static_cast<hipStream_t>(stream)),
+                "qepw_hermitian_complete");
+}
+
+int qepw_shift_scale(const void* a, int64_t lda, int n, double sigma, void* x, int64_t ldx,
+                     double* bound, void* stream) {
+  if (n < 0 || (n > 0 && (!a || !x || !bound || lda < n || ldx < n)))
+    return fail(QEPW_ERR_ARG, "qepw_shift_scale: bad argument");
+  return finish(qepw::shift_scale(a, lda, n, sigma, x, ldx, bound,
+                                  static_cast<hipStream_t>(stream)),
+                "qepw_shift_scale");
+}
+
+int qepw_hermitize(void* x, int64_t ldx, int n, void* stream) {
+  if (n < 0 || (n > 0 && (!x || ldx < n))) return fail(QEPW_ERR_ARG, "qepw_hermitize: bad argument");
+  return finish(qepw::hermitize(x, ldx, n, static_cast<hipStream_t>(stream)), "qepw_hermitize");
+}
+
+int qepw_sign_check(const void* x, int64_t ldx, const void* x2, int64_t ld2, int n, double* out,
+                    void* stream) {
+  if (n <= 0 || !x || !x2 || !out || ldx < n || ld2 < n)
+    return fail(QEPW_ERR_ARG, "qepw_sign_check: bad argument");
+  return finish(qepw::sign_check(x, ldx, x2, ld2, n, out, static_cast<hipStream_t>(stream)),
+                "qepw_sign_check");
+}
+
+int qepw_set_diagonal(void* a, int64_t lda, int n, const double* d, void* stream) {
+  if (n < 0 || (n > 0 && (!a || lda < n)))
+    return fail(QEPW_ERR_ARG, "qepw_set_diagonal: bad argument");
+  return finish(qepw::set_diagonal(a, lda, n, d, static_cast<hipStream_t>(stream)),
+                "qepw_set_diagonal");
+}
+
+}  // extern "C"

@@ -1,9 +1,10 @@
-// Complex GEMM on CDNA4 matrix cores: C = alpha op(A) B + beta C.
+// Complex GEMM on CDNA4 matrix cores: C = alpha op(A) op(B) + beta C.
 //
 // Every complex product is four real MFMA products accumulated into separate
 // real / imaginary accumulators (no 3M trick, which loses accuracy):
 //   op N : re += Ar Br - Ai Bi ;  im += Ar Bi + Ai Br
 //   op C : re += Ar Br + Ai Bi ;  im += Ar Bi - Ai Br
+// (op(B) = B^H is conjugated as it is staged into LDS).
 // using v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (FP32, FP32
 // accumulation).  A 256-thread workgroup owns a 64x64 output tile; each of
 // its four waves owns a 32x32 quarter (2x2 MFMA blocks of 16x16) and walks K
@@ -44,7 +45,7 @@ constexpr int kBK = 16;
 constexpr int kGemmThreads = 256;
 constexpr int kPad = 1;
 
-template <typename T, bool CONJ_A>
+template <typename T, bool CONJ_A, bool CONJ_B>
 __global__ __launch_bounds__(kGemmThreads) void k_cgemm(
     int64_t M, int64_t N, int64_t K, int64_t kchunk, const cplx<T>* __restrict__ A,
     int64_t lda, const cplx<T>* __restrict__ B, int64_t ldb, cplx<T>* __restrict__ C,
@@ -82,10 +83,19 @@ __global__ __launch_bounds__(kGemmThreads) void k_cgemm(
       cplx<T> v = zero;
       if (k < kend && m < M) v = CONJ_A ? A[k + m * lda] : A[m + k * lda];
       As[kk][mm] = v;
-      const int kb = e & (kBK - 1), nb = e >> 4;
+      int kb, nb;
+      if (CONJ_B) { nb = e & (kTile - 1); kb = e >> 6; }
+      else { kb = e & (kBK - 1); nb = e >> 4; }
       const int64_t kq = k0 + kb, n = n0 + nb;
       cplx<T> w = zero;
-      if (kq < kend && n < N) w = B[kq + n * ldb];
+      if (kq < kend && n < N) {
+        if (CONJ_B) {  // op(B) = B^H, B stored n x k
+          w = B[n + kq * ldb];
+          w.y = -w.y;
+        } else {
+          w = B[kq + n * ldb];
+        }
+      }
       Bs[kb][nb] = w;
     }
     __syncthreads();
@@ -172,9 +182,12 @@ struct SplitPlan {
 static SplitPlan split_plan(int op_a, int64_t m, int64_t n, int64_t k) {
   const int64_t tiles = ((m + kTile - 1) / kTile) * ((n + kTile - 1) / kTile);
   int64_t splits = 1;
-  if (op_a == QEPW_OP_C && k > 512) {
+  // split K when the output tiles alone cannot fill the 256 CUs: the tall
+  // A^H B products over npw and the small square products of the reduced
+  // eigensolve
+  if ((op_a == QEPW_OP_C && k > 512) || (tiles < 256 && k >= 256)) {
     const int64_t want = (1024 + tiles - 1) / tiles;
-    const int64_t most = k / 256;
+    const int64_t most = k / 128;
     splits = want < most ? want : most;
     if (splits < 1) splits = 1;
   }
@@ -186,18 +199,18 @@ static SplitPlan split_plan(int op_a, int64_t m, int64_t n, int64_t k) {
   return {splits, kchunk};
 }
 
-size_t gemm_workspace_bytes(int precision, int op_a, int64_t m, int64_t n, int64_t k) {
-  const SplitPlan sp = split_plan(op_a, m, n, k);
+size_t gemm_workspace_bytes(int precision, int op, int64_t m, int64_t n, int64_t k) {
+  const SplitPlan sp = split_plan(op & QEPW_OP_C, m, n, k);
   if (sp.splits <= 1) return 0;
   const size_t c = precision == QEPW_C128 ? sizeof(double2) : sizeof(float2);
   return (size_t)sp.splits * (size_t)m * (size_t)n * c;
 }
 
 template <typename T>
-static int gemm_t(int op_a, int64_t m, int64_t n, int64_t k, const double* alpha,
+static int gemm_t(int op, int64_t m, int64_t n, int64_t k, const double* alpha,
                   const void* a, int64_t lda, const void* b, int64_t ldb, const double* beta,
                   void* c, int64_t ldc, void* ws, size_t ws_bytes, hipStream_t stream) {
-  const SplitPlan sp = split_plan(op_a, m, n, k);
+  const SplitPlan sp = split_plan(op & QEPW_OP_C, m, n, k);
   if (sp.splits > 65535) return QEPW_ERR_UNSUPPORTED;
   const size_t need = sp.splits > 1 ? (size_t)sp.splits * m * n * sizeof(cplx<T>) : 0;
   if (ws_bytes < need) return QEPW_ERR_WORKSPACE;
@@ -205,16 +218,18 @@ static int gemm_t(int op_a, int64_t m, int64_t n, int64_t k, const double* alpha
                   (unsigned)sp.splits);
   const T ar = (T)alpha[0], ai = (T)alpha[1], br = (T)beta[0], bi = (T)beta[1];
   cplx<T>* part = static_cast<cplx<T>*>(ws);
-  if (op_a == QEPW_OP_C)
-    hipLaunchKernelGGL((k_cgemm<T, true>), grid, dim3(kGemmThreads), 0, stream, m, n, k,
-                       sp.kchunk, static_cast<const cplx<T>*>(a), lda,
-                       static_cast<const cplx<T>*>(b), ldb, static_cast<cplx<T>*>(c), ldc,
-                       part, ar, ai, br, bi);
-  else
-    hipLaunchKernelGGL((k_cgemm<T, false>), grid, dim3(kGemmThreads), 0, stream, m, n, k,
-                       sp.kchunk, static_cast<const cplx<T>*>(a), lda,
-                       static_cast<const cplx<T>*>(b), ldb, static_cast<cplx<T>*>(c), ldc,
-                       part, ar, ai, br, bi);
+  const cplx<T>* pa = static_cast<const cplx<T>*>(a);
+  const cplx<T>* pb = static_cast<const cplx<T>*>(b);
+  cplx<T>* pc = static_cast<cplx<T>*>(c);
+#define QEPW_GEMM_LAUNCH(CA, CB)                                                            \
+  hipLaunchKernelGGL((k_cgemm<T, CA, CB>), grid, dim3(kGemmThreads), 0, stream, m, n, k, \
+                     sp.kchunk, pa, lda, pb, ldb, pc, ldc, part, ar, ai, br, bi)
+  const bool ca = (op & QEPW_OP_C) != 0, cb = (op & QEPW_OP_BC) != 0;
+  if (ca && cb) QEPW_GEMM_LAUNCH(true, true);
+  else if (ca) QEPW_GEMM_LAUNCH(true, false);
+  else if (cb) QEPW_GEMM_LAUNCH(false, true);
+  else QEPW_GEMM_LAUNCH(false, false);
+#undef QEPW_GEMM_LAUNCH
   if (sp.splits > 1) {
     const int64_t total = m * n;
     hipLaunchKernelGGL(k_reduce_splits<T>, dim3((unsigned)((total + 255) / 256)), dim3(256),
@@ -224,21 +239,22 @@ static int gemm_t(int op_a, int64_t m, int64_t n, int64_t k, const double* alpha
   return hipGetLastError() == hipSuccess ? QEPW_OK : QEPW_ERR_HIP;
 }
 
-int gemm(int precision, int op_a, int64_t m, int64_t n, int64_t k, const double* alpha,
+int gemm(int precision, int op, int64_t m, int64_t n, int64_t k, const double* alpha,
          const void* a, int64_t lda, const void* b, int64_t ldb, const double* beta,
          void* c, int64_t ldc, void* ws, size_t ws_bytes, hipStream_t stream) {
   if (m < 0 || n < 0 || k < 0) return QEPW_ERR_ARG;
-  if (op_a != QEPW_OP_N && op_a != QEPW_OP_C) return QEPW_ERR_ARG;
+  if (op & ~(QEPW_OP_C | QEPW_OP_BC)) return QEPW_ERR_ARG;
   if (m == 0 || n == 0) return QEPW_OK;
-  if (ldc < m || ldb < (k > 0 ? k : 1)) return QEPW_ERR_ARG;
-  if (op_a == QEPW_OP_N ? lda < m : lda < (k > 0 ? k : 1)) return QEPW_ERR_ARG;
+  const bool ca = (op & QEPW_OP_C) != 0, cb = (op & QEPW_OP_BC) != 0;
+  if (ldc < m || ldb < (cb ? n : (k > 0 ? k : 1))) return QEPW_ERR_ARG;
+  if (!ca ? lda < m : lda < (k > 0 ? k : 1)) return QEPW_ERR_ARG;
   if ((m + kTile - 1) / kTile > 0x7fffffff || (n + kTile - 1) / kTile > 65535)
     return QEPW_ERR_UNSUPPORTED;
   if (precision == QEPW_C128)
-    return gemm_t<double>(op_a, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ws, ws_bytes,
+    return gemm_t<double>(op, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ws, ws_bytes,
                           stream);
   if (precision == QEPW_C64)
-    return gemm_t<float>(op_a, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ws, ws_bytes,
+    return gemm_t<float>(op, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ws, ws_bytes,
                          stream);
   return QEPW_ERR_ARG;
 }

@@ -60,4 +60,13 @@ int projector_forces(int precision, const void* dO, int64_t lddo, int64_t dstrid
                      const void* DO, int64_t lddO, const double* f, int nb, const int* off,
                      int natom, double* out, hipStream_t stream);
 
+// qepw_eig.hip
+int hermitian_complete(void* a, int64_t lda, int first, int active, hipStream_t stream);
+int shift_scale(const void* a, int64_t lda, int n, double sigma, void* x, int64_t ldx,
+                double* bound, hipStream_t stream);
+int hermitize(void* x, int64_t ldx, int n, hipStream_t stream);
+int sign_check(const void* x, int64_t ldx, const void* x2, int64_t ld2, int n, double* out,
+               hipStream_t stream);
+int set_diagonal(void* a, int64_t lda, int n, const double* d, hipStream_t stream);
+
 }  // namespace qepw

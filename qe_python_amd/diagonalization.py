@@ -20,12 +20,14 @@ Design (one GPU):
   boundary.
 * ``davidson`` keeps ``basis_storage``/``applied_storage`` on the device in
   device order.  h_psi, every subspace product (MFMA GEMMs), the residual,
-  preconditioner, norms and normalisation run as HIP kernels.  Only the
-  reduced (nact x nact) H/S rows come to the host, where the FP64
-  Cholesky + Hermitian eigensolve (LAPACK zhegvd, the same routine the
-  reference calls from ``_native_fft.pyx:667-691``) runs, and the Ritz
-  coefficients go back.  Convergence flags are host scalars, as in the
-  reference.
+  preconditioner, norms and normalisation run as HIP kernels.  The reduced
+  pencil stays on the device too (``qe_python_amd.eigensolver``): FP64
+  standardisation and a spectral-division eigensolve built from GEMMs, with
+  LAPACK zhegvd on the host (the routine the reference calls from
+  ``_native_fft.pyx:667-691``) for the first solve and as the fallback.
+  Convergence flags are host scalars, as in the reference.
+* ``precision="mixed"`` runs the Davidson in complex64 storage with FP32
+  h_psi and GEMMs and hands its Ritz vectors to an FP64 refinement.
 * Operators that are not a device ``PlaneWaveHamiltonian`` are still
   accepted: their blocks are copied to the host for each application.
 
@@ -45,6 +47,18 @@ import torch
 
 from . import _native as nat
 from . import device as dv
+from .eigensolver import DeviceRayleighRitz
+
+# default reduced eigensolve of ``davidson``: "device" (GPU spectral
+# division) or "host" (LAPACK zhegvd, the reference's routine)
+REDUCED_EIGENSOLVER = "device"
+# exit of the complex64 phase of the mixed Davidson: |de| below 1e-5 Ha and
+# residual norms below 1e-5, about ten times the FP32 noise floor of both
+# (unit roundoff 6e-8 times |H| ~ 15 Ha for a 30 Ry cutoff)
+MIXED_SWITCH = (1.0e-5, 1.0e-5)
+# a complex64 phase that has not lowered its largest unconverged residual
+# by half over this many iterations sits at the FP32 floor and hands over
+MIXED_STALL_WINDOW = 4
 
 BlockOperator = Callable[[np.ndarray], np.ndarray]
 BlockOperatorInto = Callable[[np.ndarray, np.ndarray], np.ndarray]
@@ -157,7 +171,9 @@ class PlaneWaveHamiltonian:
 
     Arguments follow ``qepy_pw/diagonalization.py:127-137``.  Keyword-only
     extras: ``device`` (HIP device, default the current one) and
-    ``precision`` ("fp64" or "fp32") of ``apply``/``apply_into``.
+    ``precision``: "fp64" or "fp32" arithmetic of ``apply``/``apply_into``,
+    or "mixed" (``apply`` in FP64; ``davidson`` on this operator runs its
+    mixed complex64 + FP64-refinement form).
     """
 
     def __init__(self, basis, potential_g, projector_terms=(), local_workspace=None,
@@ -173,8 +189,8 @@ class PlaneWaveHamiltonian:
         if getattr(self.mpi, "size", 1) > 1:
             raise ValueError("the device Hamiltonian runs on one GPU; a distributed "
                              "LocalPotentialWorkspace is not supported")
-        if precision not in ("fp64", "fp32"):
-            raise ValueError("precision must be 'fp64' or 'fp32'")
+        if precision not in ("fp64", "fp32", "mixed"):
+            raise ValueError("precision must be 'fp64', 'fp32' or 'mixed'")
         self.precision = precision
         shape = (tuple(local_workspace.shape) if local_workspace is not None
                  else tuple(self.potential_g.shape))
@@ -355,7 +371,7 @@ class PlaneWaveHamiltonian:
             return self._apply(coefficients, out=out)
 
     def _apply(self, coefficients, out=None):
-        dtype = torch.complex128 if self.precision == "fp64" else torch.complex64
+        dtype = torch.complex64 if self.precision == "fp32" else torch.complex128
         if isinstance(coefficients, torch.Tensor):
             vec = coefficients
             was_vector = vec.dim() == 1
@@ -488,13 +504,42 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
              residual_energy_scale=None, occupied_roots=None, initial_is_ritz=False,
              initial_is_orthonormal=False, initial_eigenvalues=None, initial_applied=None,
              mpi=None, global_dimension=None, global_row_indices=None, timers=None,
-             operator_into=None, device_output=False):
+             operator_into=None, device_output=False, reduced_eigensolver=None,
+             precision=None, mixed_switch=None):
     """Lowest eigenpairs by restarted block Davidson (QE ``cegterg``) on the
     GPU; semantics of ``qepy_pw/diagonalization.py:527-880``.
 
-    Extra keyword: ``device_output=True`` returns the eigenvectors as a
-    device tensor (npw, nroots) in reference row order instead of a host
-    array (no device-to-host copy of the vectors)."""
+    Extra keywords:
+
+    * ``device_output=True`` returns the eigenvectors as a device tensor
+      (npw, nroots) in reference row order instead of a host array (no
+      device-to-host copy of the vectors).
+    * ``reduced_eigensolver`` selects the reduced eigensolve: "device"
+      (spectral division on the GPU, the default,
+      ``qe_python_amd.eigensolver``) or "host" (LAPACK zhegvd on the copied
+      pencil, as the reference does).
+    * ``precision``: "fp64" (complex128 storage; FP64 h_psi and GEMMs) or
+      "mixed".  The mixed form runs a complex64 phase -- FP32 FFT, V*psi,
+      projections and subspace GEMMs on FP32 MFMA with FP32 accumulation,
+      while the Cholesky, the reduced eigensolve and the residual norms stay
+      FP64 -- until every root has |de| < ``mixed_switch[0]`` Ha and a
+      residual norm < ``mixed_switch[1]`` (or the phase stalls at the FP32
+      noise floor), then an FP64 refinement: FP64 h_psi of the complex64
+      Ritz vectors, Rayleigh-Ritz and FP64 Davidson iterations to
+      ``tolerance``.  ``None`` selects "mixed" when the operator is a
+      ``PlaneWaveHamiltonian`` built with ``precision="mixed"``, else "fp64".
+      The result's ``iterations`` and ``hamiltonian_applications`` count
+      both phases.
+    """
+    if reduced_eigensolver is None:
+        reduced_eigensolver = REDUCED_EIGENSOLVER
+    if reduced_eigensolver not in ("device", "host"):
+        raise ValueError("reduced_eigensolver must be 'device' or 'host'")
+    if precision not in (None, "fp64", "mixed"):
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    switch = MIXED_SWITCH if mixed_switch is None else tuple(float(x) for x in mixed_switch)
+    if len(switch) != 2 or min(switch) <= 0.0:
+        raise ValueError("mixed_switch must be two positive numbers (|de| in Ha, residual norm)")
     diagonal = np.asarray(diagonal, dtype=float)
     mpi = mpi if mpi is not None else _SERIAL
     if getattr(mpi, "size", 1) > 1:
@@ -548,74 +593,159 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
         perm_dev, perm = None, None
         op = _HostOperator(operator, operator_into, npw, device)
         ws = dv.Workspace(device)
-    dtype = torch.complex128
+    if precision is None:
+        precision = "mixed" if (ham is not None and ham.precision == "mixed") else "fp64"
     diag_dev = torch.from_numpy(np.ascontiguousarray(
         diagonal if perm is None else diagonal[perm])).to(device)
+    ctx = _PhaseContext(op, ws, device, npw, diag_dev, timers, number_of_roots, dimension,
+                        subspace_multiplier, reduced_eigensolver == "device")
+    energy_bounds = root_tolerances
+    residual_bounds = None if residual_factor is None else residual_factor * root_tolerances
 
-    trial_dev = _upload_block(trial, perm_dev, npw, device, dtype)
-    if initial_is_ritz or initial_is_orthonormal:
-        basis = trial_dev[:number_of_roots]
+    def applied_block(active):
+        if initial_applied is None:
+            return None
+        if tuple(initial_applied.shape) != (npw, active):
+            raise ValueError("initial_applied must match the initial Ritz block")
+        return initial_applied
+
+    if precision == "fp64":
+        dtype = torch.complex128
+        start = _upload_block(trial, perm_dev, npw, device, dtype)
+        phase = _davidson_phase(
+            ctx, start, dtype, energy_bounds, residual_bounds, residual_energy_scale,
+            root_tolerances, max_iterations, initial_is_ritz, initial_is_orthonormal,
+            initial_eigenvalues, applied_block, perm_dev)
+        return DavidsonResult(phase.values, _result_vectors(phase.X, perm_dev, npw, device_output),
+                              phase.residual_norms, phase.iterations, phase.converged,
+                              phase.number_unconverged, phase.happ)
+
+    # mixed: complex64 phase to the switch thresholds ...
+    de_switch, res_switch = switch
+    start = _upload_block(trial, perm_dev, npw, device, torch.complex64)
+    coarse = _davidson_phase(
+        ctx, start, torch.complex64, np.maximum(energy_bounds, de_switch),
+        np.full(number_of_roots, res_switch), None, np.maximum(root_tolerances, de_switch),
+        max_iterations, initial_is_ritz, initial_is_orthonormal, initial_eigenvalues,
+        applied_block, perm_dev, stall_window=MIXED_STALL_WINDOW, require_residuals=True)
+    del start
+    # ... then the FP64 refinement from its Ritz vectors; the reduced pencil
+    # carries S, so the FP32-orthonormal block needs no re-orthonormalisation
+    start = coarse.X.to(torch.complex128)
+    coarse.X = None
+    fine = _davidson_phase(
+        ctx, start, torch.complex128, energy_bounds, residual_bounds, residual_energy_scale,
+        root_tolerances, max_iterations, False, True, None, lambda active: None, perm_dev)
+    return DavidsonResult(fine.values, _result_vectors(fine.X, perm_dev, npw, device_output),
+                          fine.residual_norms, coarse.iterations + fine.iterations,
+                          fine.converged, fine.number_unconverged, coarse.happ + fine.happ)
+
+
+@dataclass
+class _PhaseContext:
+    op: object
+    ws: dv.Workspace
+    device: torch.device
+    npw: int
+    diag_dev: torch.Tensor
+    timers: object
+    nroots: int
+    dimension: int
+    subspace_multiplier: int
+    spectral: bool
+
+
+@dataclass
+class _PhaseResult:
+    values: np.ndarray
+    X: torch.Tensor | None
+    residual_norms: np.ndarray
+    iterations: int
+    converged: bool
+    number_unconverged: int
+    happ: int
+
+
+def _davidson_phase(ctx, start, dtype, energy_bounds, residual_bounds, residual_energy_scale,
+                    root_tolerances, max_iterations, start_is_ritz, start_is_orthonormal,
+                    initial_eigenvalues, applied_block, perm_dev, *, stall_window=None,
+                    require_residuals=False):
+    """The restarted Davidson loop (``diagonalization.py:640-880``) at one
+    storage dtype, from a device-order start block ``start`` (nb, npw).
+
+    A root converges when |de| < ``energy_bounds`` and, when given, its
+    residual norm < ``residual_bounds`` and its squared residual norm <
+    ``residual_energy_scale * root_tolerances``.  ``stall_window``: stop
+    unconverged when the largest unconverged residual norm has not halved
+    over that many iterations."""
+    op, ws, device, npw, timers = ctx.op, ctx.ws, ctx.device, ctx.npw, ctx.timers
+    number_of_roots = ctx.nroots
+    if start_is_ritz or start_is_orthonormal:
+        basis = start[:number_of_roots]
     else:
         with _measure(timers, "cegterg:over", device):
-            basis = _orthonormalize_device(trial_dev, npw, ws)
+            basis = _orthonormalize_device(start, npw, ws)
     if basis.shape[0] < number_of_roots:
         raise ValueError("initial Davidson vectors are linearly dependent")
-    maximum_subspace = min(dimension, max(basis.shape[0], 2 * number_of_roots,
-                                          subspace_multiplier * number_of_roots))
+    maximum_subspace = min(ctx.dimension, max(basis.shape[0], 2 * number_of_roots,
+                                              ctx.subspace_multiplier * number_of_roots))
     active = int(basis.shape[0])
     V = torch.empty((maximum_subspace, npw), dtype=dtype, device=device)
     HV = torch.empty_like(V)
     V[:active].copy_(basis)
-    del basis, trial_dev
-    if initial_applied is None:
+    del basis, start
+    given = applied_block(active)
+    if given is None:
         op.device_apply(V[:active], HV[:active])
         happ = active
     else:
-        shape = tuple(initial_applied.shape)
-        if shape != (npw, active):
-            raise ValueError("initial_applied must match the initial Ritz block")
-        HV[:active].copy_(_upload_block(initial_applied, perm_dev, npw, device, dtype))
+        HV[:active].copy_(_upload_block(given, perm_dev, npw, device, dtype))
         happ = 0
-    projected = np.zeros((maximum_subspace, maximum_subspace), dtype=complex)
-    overlap = np.zeros_like(projected)
+    # the reduced pencil Hs = V^H HV, Ss = V^H V lives on the device
+    rr = DeviceRayleighRitz(maximum_subspace, number_of_roots, device, ws,
+                            spectral=ctx.spectral)
+    diag_dev = ctx.diag_dev
 
     def update_projected(first):
         with _measure(timers, "cegterg:over", device):
-            nnew = active - first
-            rows = torch.empty((2, active, nnew), dtype=dtype, device=device)
-            dv.gemm(nat.OP_C, nnew, active, npw, 1.0, V[first:active], HV[:active], 0.0,
-                    rows[0], ws)
-            dv.gemm(nat.OP_C, nnew, active, npw, 1.0, V[first:active], V[:active], 0.0,
-                    rows[1], ws)
-            host = rows.cpu().numpy()
-            h_rows = host[0].T
-            s_rows = host[1].T
-            projected[first:active, :active] = h_rows
-            overlap[first:active, :active] = s_rows
-            if first:
-                projected[:first, first:active] = h_rows[:, :first].conj().T
-                overlap[:first, first:active] = s_rows[:, :first].conj().T
-            sl = slice(first, active)
-            projected[sl, sl] = 0.5 * (projected[sl, sl] + projected[sl, sl].conj().T)
-            overlap[sl, sl] = 0.5 * (overlap[sl, sl] + overlap[sl, sl].conj().T)
+            rr.update(V, HV, first, active, npw)
 
     update_projected(0)
 
-    def ritz_pairs():
-        with _measure(timers, "cdiaghg"):
-            roots, coefficients = _lowest_generalized_eigh(
-                projected[:active, :active], overlap[:active, :active], number_of_roots)
+    def ritz_solve():
+        with _measure(timers, "cdiaghg", device):
+            roots, coef = rr.solve(active)
+            coef = coef.to(dtype)
+        return roots, coef
+
+    def ritz_block(roots, coef):
+        """Ritz vectors X = V C and residuals R = HV C - X diag(e)."""
         with _measure(timers, "cegterg:upda", device):
-            coef = torch.from_numpy(np.ascontiguousarray(coefficients.T)).to(device)
             X = torch.empty((number_of_roots, npw), dtype=dtype, device=device)
             R = torch.empty_like(X)
             dv.gemm(nat.OP_N, npw, number_of_roots, active, 1.0, V[:active], coef, 0.0, X, ws)
             dv.gemm(nat.OP_N, npw, number_of_roots, active, 1.0, HV[:active], coef, 0.0, R, ws)
             e = torch.from_numpy(np.ascontiguousarray(roots, dtype=np.float64)).to(device)
             dv.subtract_band_energies(R, X, e, npw)
-        return roots, X, R
+        return X, R
 
-    if initial_is_ritz:
+    def selected_residuals(roots, coef, cols):
+        """Residuals of the roots ``cols`` only, packed:
+        R_u = HV C_u - V (C_u diag(e_u)) -- all the next expansion needs."""
+        with _measure(timers, "cegterg:upda", device):
+            idx = torch.from_numpy(cols.astype(np.int64)).to(device)
+            cu = coef.index_select(0, idx)
+            e = torch.from_numpy(np.ascontiguousarray(roots[cols], dtype=np.float64)).to(device)
+            ce = cu * e.to(cu.real.dtype)[:, None]
+            R = torch.empty((len(cols), npw), dtype=dtype, device=device)
+            dv.gemm(nat.OP_N, npw, len(cols), active, 1.0, HV[:active], cu, 0.0, R, ws)
+            dv.gemm(nat.OP_N, npw, len(cols), active, -1.0, V[:active], ce, 1.0, R, ws)
+        return R
+
+    track = (residual_bounds is not None or residual_energy_scale is not None
+             or require_residuals)
+    coef = None
+    if start_is_ritz:
         if initial_eigenvalues is None:
             ip = dv.column_inner_products(V[:number_of_roots], HV[:number_of_roots], npw)
             values = ip.cpu().numpy()[0::2].copy()
@@ -626,8 +756,8 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
         dv.subtract_band_energies(R, X, torch.from_numpy(np.ascontiguousarray(values)).to(device),
                                   npw)
     else:
-        values, X, R = ritz_pairs()
-    track = residual_factor is not None or residual_energy_scale is not None
+        values, coef = ritz_solve()
+        X, R = ritz_block(values, coef)
 
     def residual_norms_of(block):
         sq = dv.column_squared_norms(block, npw).cpu().numpy()
@@ -637,15 +767,21 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
     completed = 0
     number_unconverged = number_of_roots
     unconverged = np.ones(number_of_roots, dtype=bool)
+    # residual rows of the unconverged roots for the next expansion: R holds
+    # all roots (rows ``cols``) or, without residual tracking, only them
+    # (rows 0..len(cols)-1)
+    packed = False
+    progress = []
     for iteration in range(1, max_iterations + 1):
         completed = iteration
         previous = values
         with _measure(timers, "g_psi", device):
             cols = np.flatnonzero(unconverged).astype(np.int32)
-            cols_dev = torch.from_numpy(cols).to(device)
+            rows = np.arange(len(cols), dtype=np.int32) if packed else cols
+            rows_dev = torch.from_numpy(rows).to(device)
             e_dev = torch.from_numpy(np.ascontiguousarray(values[cols])).to(device)
             corrections = torch.empty((len(cols), npw), dtype=dtype, device=device)
-            dv.qe_precondition(R, cols_dev, e_dev, diag_dev, corrections, npw)
+            dv.qe_precondition(R, rows_dev, e_dev, diag_dev, corrections, npw)
         # _normalize_columns (diagonalization.py:500-524): no orthogonalisation
         sq_dev = dv.column_squared_norms(corrections, npw)
         sq = sq_dev.cpu().numpy()
@@ -660,37 +796,50 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
         dv.normalize_selected_columns(corrections, sq_dev, torch.from_numpy(keep).to(device),
                                       V[old:active], npw)
         del X, R, corrections
+        X = None
         op.device_apply(V[old:active], HV[old:active])
         happ += len(keep)
         update_projected(old)
-        values, X, R = ritz_pairs()
+        values, coef = ritz_solve()
         if track:
+            X, R = ritz_block(values, coef)
+            packed = False
             residual_norms = residual_norms_of(R)
-        converged = np.abs(values - previous) < root_tolerances
-        if residual_factor is not None:
-            converged &= residual_norms < residual_factor * root_tolerances
+        converged = np.abs(values - previous) < energy_bounds
+        if residual_bounds is not None:
+            converged &= residual_norms < residual_bounds
         if residual_energy_scale is not None:
             converged &= residual_norms ** 2 < residual_energy_scale * root_tolerances
         unconverged = ~converged
         number_unconverged = int(np.count_nonzero(unconverged))
         if number_unconverged == 0:
             if not track:
+                X, R = ritz_block(values, coef)
                 residual_norms = residual_norms_of(R)
-            return DavidsonResult(values, _result_vectors(X, perm_dev, npw, device_output), residual_norms,
-                                  iteration, True, 0, happ)
+            return _PhaseResult(values, X, residual_norms, iteration, True, 0, happ)
+        if stall_window:
+            progress.append(float(np.max(residual_norms[unconverged])))
+            if (len(progress) > stall_window
+                    and min(progress[-stall_window:]) > 0.5 * min(progress[:-stall_window])):
+                break
         if active + number_unconverged > maximum_subspace:
             with _measure(timers, "cegterg:last", device):
+                if X is None:
+                    X, R = ritz_block(values, coef)
+                    packed = False
                 active = number_of_roots
                 V[:active].copy_(X)
                 e = torch.from_numpy(np.ascontiguousarray(values)).to(device)
                 dv.restart_applied(X, R, e, HV[:active], npw)
-                projected[:active, :active] = 0.0
-                projected[np.arange(active), np.arange(active)] = values
-                overlap[:active, :active] = np.eye(active, dtype=complex)
+                rr.restart(values)
+        elif X is None:
+            R = selected_residuals(values, coef, np.flatnonzero(unconverged))
+            packed = True
+    if X is None or packed:
+        X, R = ritz_block(values, coef)
     if not track:
         residual_norms = residual_norms_of(R)
-    return DavidsonResult(values, _result_vectors(X, perm_dev, npw, device_output), residual_norms, completed,
-                          False, number_unconverged, happ)
+    return _PhaseResult(values, X, residual_norms, completed, False, number_unconverged, happ)
 
 
 def _result_vectors(X, perm_dev, npw, device_output=False):

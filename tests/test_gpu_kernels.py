@@ -32,7 +32,7 @@ def rand(rng, *shape):
 
 
 @pytest.mark.parametrize("prec", ["c128", "c64"])
-@pytest.mark.parametrize("op", ["N", "C"])
+@pytest.mark.parametrize("op", ["N", "C", "NB", "CB"])
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (5, 7, 3), (64, 64, 16), (65, 130, 37),
                                    (256, 128, 23847), (17, 300, 4000), (2945, 8, 32),
                                    (3, 2, 0)])
@@ -41,14 +41,16 @@ def test_gemm_matches_numpy(prec, op, m, n, k):
     from qe_python_amd import device as dv
     rng = np.random.default_rng(m * 1000 + n * 10 + k)
     dtype = torch.complex128 if prec == "c128" else torch.complex64
-    a = rand(rng, m, k) if op == "N" else rand(rng, k, m)
-    b = rand(rng, k, n)
+    a = rand(rng, m, k) if op[0] == "N" else rand(rng, k, m)
+    b = rand(rng, n, k) if op.endswith("B") else rand(rng, k, n)
     c = rand(rng, m, n)
     alpha, beta = 0.7 - 0.2j, -0.3 + 0.5j
     ta, tb, tc = dev_block(a, dtype), dev_block(b, dtype), dev_block(c, dtype)
     ws = dv.Workspace(torch.device(DEV))
-    dv.gemm(nat.OP_N if op == "N" else nat.OP_C, m, n, k, alpha, ta, tb, beta, tc, ws)
-    opa = a if op == "N" else a.conj().T
+    code = (nat.OP_N if op[0] == "N" else nat.OP_C) | (nat.OP_BC if op.endswith("B") else 0)
+    dv.gemm(code, m, n, k, alpha, ta, tb, beta, tc, ws)
+    opa = a if op[0] == "N" else a.conj().T
+    b = b.conj().T if op.endswith("B") else b
     if prec == "c64":
         a32, b32, c32 = (x.astype(np.complex64).astype(np.complex128) for x in (opa, b, c))
         ref = alpha * (a32 @ b32) + beta * c32
