@@ -254,9 +254,11 @@ int qepw_projector_forces(int precision, const void* dO, int64_t lddo, int64_t d
  *   columns [0, active); mirror them into the upper block and symmetrise the
  *   new diagonal block (update_projected, diagonalization.py:658-697).
  * qepw_shift_scale: bound[0] = max_i sum_j |A_ij - sigma delta_ij| (device
- *   double, an upper bound of ||A - sigma I||_2) and X = (A - sigma I)/bound.
+ *   double, an upper bound of ||A - sigma I||_2) and X = (A - sigma I)/bound;
+ *   ``bound`` holds n + 1 doubles (bound[1 + j] = column sums).
  * qepw_hermitize: X = (X + X^H)/2 in place.
- * qepw_sign_check: out[0] = ||X2 - I||_F^2, out[1] = Re trace(X) (device).
+ * qepw_sign_check: out[0] = ||X2 - I||_F^2, out[1] = Re trace(X) (device);
+ *   ``out`` holds n + 2 doubles (out[2 + j] = per-column partial sums).
  * qepw_set_diagonal: A[:n, :n] = diag(d), identity when d is NULL.
  */
 int qepw_hermitian_complete(void* a, int64_t lda, int first, int active, void* stream);

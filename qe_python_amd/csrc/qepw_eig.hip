@@ -39,39 +39,52 @@ __global__ void k_hermitian_complete(double2* __restrict__ a, int64_t lda, int f
   }
 }
 
-// out[0] = max_i sum_j |A_ij - sigma delta_ij|  (>= ||A - sigma I||_2), one block
-__global__ __launch_bounds__(kEigThreads) void k_shift_bound(const double2* __restrict__ a,
-                                                             int64_t lda, int n, double sigma,
-                                                             double* __restrict__ out) {
-  __shared__ double sh[kEigThreads];
-  double best = 0.0;
-  for (int i = threadIdx.x; i < n; i += kEigThreads) {
-    double s = 0.0;
-    for (int j = 0; j < n; ++j) {
-      double2 v = a[i + (int64_t)j * lda];
-      if (i == j) v.x -= sigma;
-      s += sqrt(v.x * v.x + v.y * v.y);
-    }
-    best = s > best ? s : best;
-  }
-  sh[threadIdx.x] = best;
-  __syncthreads();
-  for (int s = kEigThreads / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s && sh[threadIdx.x + s] > sh[threadIdx.x])
-      sh[threadIdx.x] = sh[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = sh[0] > 0.0 ? sh[0] : 1.0;
+// Sum over one column of a wave64: lane 0 ends with the total.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
-// X = (A - sigma I) / bound[0]
-__global__ void k_shift_scale(const double2* __restrict__ a, int64_t lda, int n, double sigma,
-                              const double* __restrict__ bound, double2* __restrict__ x,
-                              int64_t ldx) {
+// colsum[j] = sum_i |A_ij - sigma delta_ij|: one wave per column (A is
+// Hermitian, so these are also the row sums)
+__global__ __launch_bounds__(kEigThreads) void k_shift_colsums(const double2* __restrict__ a,
+                                                               int64_t lda, int n, double sigma,
+                                                               double* __restrict__ colsum) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * (kEigThreads / 64) + (threadIdx.x >> 6);
+  if (j >= n) return;
+  double s = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    double2 v = a[i + (int64_t)j * lda];
+    if (i == j) v.x -= sigma;
+    s += sqrt(v.x * v.x + v.y * v.y);
+  }
+  s = wave_sum(s);
+  if (lane == 0) colsum[j] = s;
+}
+
+// X = (A - sigma I) / bound[0], bound[0] = max_j colsum[j] (>= ||A - sigma I||_2)
+// with colsum = bound + 1; every block forms the maximum itself
+__global__ __launch_bounds__(256) void k_shift_scale(const double2* __restrict__ a,
+                                                     int64_t lda, int n, double sigma,
+                                                     double* __restrict__ bound,
+                                                     double2* __restrict__ x, int64_t ldx) {
+  __shared__ double sh[256];
+  double best = 0.0;
+  for (int j = threadIdx.x; j < n; j += 256) best = fmax(best, bound[1 + j]);
+  sh[threadIdx.x] = best;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  const double b = sh[0] > 0.0 ? sh[0] : 1.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) bound[0] = b;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)n * n) return;
   const int i = (int)(idx % n), j = (int)(idx / n);
-  const double inv = 1.0 / bound[0];
+  const double inv = 1.0 / b;
   double2 v = a[i + (int64_t)j * lda];
   if (i == j) v.x -= sigma;
   x[i + (int64_t)j * ldx] = make_double2(v.x * inv, v.y * inv);
@@ -94,20 +107,32 @@ __global__ void k_hermitize(double2* __restrict__ x, int64_t ldx, int n) {
   x[j + (int64_t)i * ldx] = make_double2(m.x, -m.y);
 }
 
-// out[0] = ||X2 - I||_F^2, out[1] = Re trace(X); one block, fixed order
-__global__ __launch_bounds__(kEigThreads) void k_sign_check(const double2* __restrict__ x,
-                                                            int64_t ldx,
-                                                            const double2* __restrict__ x2,
-                                                            int64_t ld2, int n,
+// part[j] = sum_i |X2_ij - delta_ij|^2: one wave per column
+__global__ __launch_bounds__(kEigThreads) void k_sign_colsums(const double2* __restrict__ x2,
+                                                              int64_t ld2, int n,
+                                                              double* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * (kEigThreads / 64) + (threadIdx.x >> 6);
+  if (j >= n) return;
+  double r = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    double2 v = x2[i + (int64_t)j * ld2];
+    if (i == j) v.x -= 1.0;
+    r += v.x * v.x + v.y * v.y;
+  }
+  r = wave_sum(r);
+  if (lane == 0) part[j] = r;
+}
+
+// out[0] = sum_j part[j] = ||X2 - I||_F^2, out[1] = Re trace(X) with
+// part = out + 2; one block, fixed order
+__global__ __launch_bounds__(kEigThreads) void k_sign_final(const double2* __restrict__ x,
+                                                            int64_t ldx, int n,
                                                             double* __restrict__ out) {
   __shared__ double sh[2][kEigThreads];
   double r = 0.0, t = 0.0;
   for (int j = threadIdx.x; j < n; j += kEigThreads) {
-    for (int i = 0; i < n; ++i) {
-      double2 v = x2[i + (int64_t)j * ld2];
-      if (i == j) v.x -= 1.0;
-      r += v.x * v.x + v.y * v.y;
-    }
+    r += out[2 + j];
     t += x[j + (int64_t)j * ldx].x;
   }
   sh[0][threadIdx.x] = r;
@@ -149,8 +174,9 @@ int hermitian_complete(void* a, int64_t lda, int first, int active, hipStream_t 
 int shift_scale(const void* a, int64_t lda, int n, double sigma, void* x, int64_t ldx,
                 double* bound, hipStream_t stream) {
   if (n <= 0) return QEPW_OK;
-  hipLaunchKernelGGL(k_shift_bound, dim3(1), dim3(kEigThreads), 0, stream,
-                     static_cast<const double2*>(a), lda, n, sigma, bound);
+  const unsigned col_blocks = (unsigned)((n + kEigThreads / 64 - 1) / (kEigThreads / 64));
+  hipLaunchKernelGGL(k_shift_colsums, dim3(col_blocks), dim3(kEigThreads), 0, stream,
+                     static_cast<const double2*>(a), lda, n, sigma, bound + 1);
   hipLaunchKernelGGL(k_shift_scale, dim3(blocks_for((int64_t)n * n)), dim3(256), 0, stream,
                      static_cast<const double2*>(a), lda, n, sigma, bound,
                      static_cast<double2*>(x), ldx);
@@ -166,9 +192,12 @@ int hermitize(void* x, int64_t ldx, int n, hipStream_t stream) {
 
 int sign_check(const void* x, int64_t ldx, const void* x2, int64_t ld2, int n, double* out,
                hipStream_t stream) {
-  hipLaunchKernelGGL(k_sign_check, dim3(1), dim3(kEigThreads), 0, stream,
-                     static_cast<const double2*>(x), ldx, static_cast<const double2*>(x2), ld2,
-                     n, out);
+  if (n <= 0) return QEPW_OK;
+  const unsigned col_blocks = (unsigned)((n + kEigThreads / 64 - 1) / (kEigThreads / 64));
+  hipLaunchKernelGGL(k_sign_colsums, dim3(col_blocks), dim3(kEigThreads), 0, stream,
+                     static_cast<const double2*>(x2), ld2, n, out + 2);
+  hipLaunchKernelGGL(k_sign_final, dim3(1), dim3(kEigThreads), 0, stream,
+                     static_cast<const double2*>(x), ldx, n, out);
   return status();
 }
 

@@ -54,6 +54,9 @@ def require_device(device=None) -> torch.device:
 def ld_of(t: torch.Tensor) -> int:
     if t.dim() != 2 or (t.shape[0] > 1 and t.shape[1] > 1 and t.stride(1) != 1):
         raise ValueError("device matrix must be a (ncols, ld) tensor with unit inner stride")
+    if t.is_conj() or t.is_neg():
+        # a lazy conjugate / negative view: its memory holds the plain values
+        raise ValueError("device matrix is a lazy conj/neg view; resolve it first")
     if t.shape[0] > 1:
         return max(1, int(t.stride(0)))
     return max(1, int(t.shape[1]), int(t.stride(0)))

@@ -47,7 +47,7 @@ import torch
 
 from . import _native as nat
 from . import device as dv
-from .eigensolver import DeviceRayleighRitz
+from .eigensolver import DeviceRayleighRitz, _one_blas_thread
 
 # default reduced eigensolve of ``davidson``: "device" (GPU spectral
 # division) or "host" (LAPACK zhegvd, the reference's routine)
@@ -56,9 +56,16 @@ REDUCED_EIGENSOLVER = "device"
 # residual norms below 1e-5, about ten times the FP32 noise floor of both
 # (unit roundoff 6e-8 times |H| ~ 15 Ha for a 30 Ry cutoff)
 MIXED_SWITCH = (1.0e-5, 1.0e-5)
-# a complex64 phase that has not lowered its largest unconverged residual
-# by half over this many iterations sits at the FP32 floor and hands over
+# a complex64 phase whose largest unconverged residual is within 100x of the
+# switch residual and has not halved over this many iterations sits at the
+# FP32 floor and hands over (far above the floor, slow progress is the
+# ordinary slow tail of the top roots and the cheap phase continues)
 MIXED_STALL_WINDOW = 4
+MIXED_STALL_SPAN = 100.0
+# diagnostics: set to a list to receive one record per Davidson iteration
+# (storage dtype, iteration, unconverged roots, max |de|, max residual norm
+# or None when residuals are not formed)
+ITERATION_TRACE = None
 
 BlockOperator = Callable[[np.ndarray], np.ndarray]
 BlockOperatorInto = Callable[[np.ndarray, np.ndarray], np.ndarray]
@@ -421,7 +428,7 @@ def _upload_block(vectors, perm, npw, device, dtype):
         src = t.T.to(device)
         if src.dtype not in (torch.complex128, torch.complex64):
             src = src.to(torch.complex128)
-        src = src.contiguous()
+        src = src.resolve_conj().resolve_neg().contiguous()
     else:
         host = np.asarray(vectors)
         if host.ndim != 2 or host.shape[0] != npw:
@@ -486,7 +493,8 @@ def _orthonormalize_device(block: torch.Tensor, npw: int, ws, threshold: float =
     dv.gemm(nat.OP_C, n, n, npw, 1.0, block, block, 0.0, gram_t, ws)
     gram = gram_t.cpu().numpy().T.astype(np.complex128)
     gram = 0.5 * (gram + gram.conj().T)
-    squared, rotation = np.linalg.eigh(gram)
+    with _one_blas_thread():
+        squared, rotation = np.linalg.eigh(gram)
     order = np.argsort(squared)[::-1]
     squared = np.maximum(np.real(squared[order]), 0.0)
     if not len(squared) or squared[0] == 0.0:
@@ -627,7 +635,8 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
         ctx, start, torch.complex64, np.maximum(energy_bounds, de_switch),
         np.full(number_of_roots, res_switch), None, np.maximum(root_tolerances, de_switch),
         max_iterations, initial_is_ritz, initial_is_orthonormal, initial_eigenvalues,
-        applied_block, perm_dev, stall_window=MIXED_STALL_WINDOW, require_residuals=True)
+        applied_block, perm_dev, stall_window=MIXED_STALL_WINDOW, require_residuals=True,
+        coarse=True)
     del start
     # ... then the FP64 refinement from its Ritz vectors; the reduced pencil
     # carries S, so the FP32-orthonormal block needs no re-orthonormalisation
@@ -669,14 +678,20 @@ class _PhaseResult:
 def _davidson_phase(ctx, start, dtype, energy_bounds, residual_bounds, residual_energy_scale,
                     root_tolerances, max_iterations, start_is_ritz, start_is_orthonormal,
                     initial_eigenvalues, applied_block, perm_dev, *, stall_window=None,
-                    require_residuals=False):
+                    require_residuals=False, coarse=False):
     """The restarted Davidson loop (``diagonalization.py:640-880``) at one
     storage dtype, from a device-order start block ``start`` (nb, npw).
+
+    ``coarse``: the complex64 phase of the mixed form.  When its FP32
+    overlap loses definiteness (new corrections dependent at FP32
+    precision) the phase ends on the previous Ritz pairs instead of raising
+    ``LinAlgError``; the FP64 refinement continues from them.
 
     A root converges when |de| < ``energy_bounds`` and, when given, its
     residual norm < ``residual_bounds`` and its squared residual norm <
     ``residual_energy_scale * root_tolerances``.  ``stall_window``: stop
-    unconverged when the largest unconverged residual norm has not halved
+    unconverged when the largest unconverged residual norm, already below
+    ``MIXED_STALL_SPAN`` times the largest residual bound, has not halved
     over that many iterations."""
     op, ws, device, npw, timers = ctx.op, ctx.ws, ctx.device, ctx.npw, ctx.timers
     number_of_roots = ctx.nroots
@@ -755,6 +770,8 @@ def _davidson_phase(ctx, start, dtype, energy_bounds, residual_bounds, residual_
         R = HV[:number_of_roots].clone()
         dv.subtract_band_energies(R, X, torch.from_numpy(np.ascontiguousarray(values)).to(device),
                                   npw)
+        # the start block is its own Ritz block: coefficients = identity
+        coef = torch.eye(number_of_roots, dtype=dtype, device=device)
     else:
         values, coef = ritz_solve()
         X, R = ritz_block(values, coef)
@@ -782,25 +799,55 @@ def _davidson_phase(ctx, start, dtype, energy_bounds, residual_bounds, residual_
             e_dev = torch.from_numpy(np.ascontiguousarray(values[cols])).to(device)
             corrections = torch.empty((len(cols), npw), dtype=dtype, device=device)
             dv.qe_precondition(R, rows_dev, e_dev, diag_dev, corrections, npw)
-        # _normalize_columns (diagonalization.py:500-524): no orthogonalisation
-        sq_dev = dv.column_squared_norms(corrections, npw)
-        sq = sq_dev.cpu().numpy()
-        if not len(sq):
-            break
-        scale = max(float(np.max(sq)), 1.0e-300)
-        keep = np.flatnonzero(sq > (1.0e-14) ** 2 * scale).astype(np.int32)
-        if not len(keep):
-            break
-        old = active
-        active += len(keep)
-        dv.normalize_selected_columns(corrections, sq_dev, torch.from_numpy(keep).to(device),
-                                      V[old:active], npw)
+        if coarse and len(cols):
+            # complex64 phase: the block is made orthonormal to the basis (two
+            # classical Gram-Schmidt passes) and within itself (canonical
+            # orthonormalisation dropping directions below 1e-4 of the
+            # largest) -- the same span, so the same Rayleigh-Ritz, with an
+            # FP32 overlap near I that stays positive definite
+            with _measure(timers, "cegterg:orth", device):
+                proj = torch.empty((len(cols), active), dtype=dtype, device=device)
+                for _ in range(2):
+                    dv.gemm(nat.OP_C, active, len(cols), npw, 1.0, V[:active], corrections,
+                            0.0, proj, ws)
+                    dv.gemm(nat.OP_N, npw, len(cols), active, -1.0, V[:active], proj, 1.0,
+                            corrections, ws)
+                block = _orthonormalize_device(corrections, npw, ws, threshold=1.0e-4)
+            if not block.shape[0]:
+                break
+            old = active
+            active += int(block.shape[0])
+            V[old:active].copy_(block)
+            keep = np.arange(block.shape[0])
+            del block
+        else:
+            # _normalize_columns (diagonalization.py:500-524): no orthogonalisation
+            sq_dev = dv.column_squared_norms(corrections, npw)
+            sq = sq_dev.cpu().numpy()
+            if not len(sq):
+                break
+            scale = max(float(np.max(sq)), 1.0e-300)
+            keep = np.flatnonzero(sq > (1.0e-14) ** 2 * scale).astype(np.int32)
+            if not len(keep):
+                break
+            old = active
+            active += len(keep)
+            dv.normalize_selected_columns(corrections, sq_dev, torch.from_numpy(keep).to(device),
+                                          V[old:active], npw)
         del X, R, corrections
         X = None
         op.device_apply(V[old:active], HV[old:active])
         happ += len(keep)
         update_projected(old)
-        values, coef = ritz_solve()
+        coef_prev = coef
+        try:
+            values, coef = ritz_solve()
+        except np.linalg.LinAlgError:
+            if not coarse:
+                raise
+            active, values, coef = old, previous, coef_prev
+            packed = False
+            break
         if track:
             X, R = ritz_block(values, coef)
             packed = False
@@ -812,14 +859,21 @@ def _davidson_phase(ctx, start, dtype, energy_bounds, residual_bounds, residual_
             converged &= residual_norms ** 2 < residual_energy_scale * root_tolerances
         unconverged = ~converged
         number_unconverged = int(np.count_nonzero(unconverged))
+        if ITERATION_TRACE is not None:
+            ITERATION_TRACE.append((str(dtype), iteration, number_unconverged,
+                                    float(np.max(np.abs(values - previous))),
+                                    float(np.max(residual_norms)) if track else None))
         if number_unconverged == 0:
             if not track:
                 X, R = ritz_block(values, coef)
                 residual_norms = residual_norms_of(R)
             return _PhaseResult(values, X, residual_norms, iteration, True, 0, happ)
         if stall_window:
-            progress.append(float(np.max(residual_norms[unconverged])))
-            if (len(progress) > stall_window
+            worst = float(np.max(residual_norms[unconverged]))
+            progress.append(worst)
+            near_floor = (residual_bounds is not None
+                          and worst < MIXED_STALL_SPAN * float(np.max(residual_bounds)))
+            if (near_floor and len(progress) > stall_window
                     and min(progress[-stall_window:]) > 0.5 * min(progress[:-stall_window])):
                 break
         if active + number_unconverged > maximum_subspace:

@@ -20,9 +20,17 @@ iteration with LAPACK zhegvd on the host (``generalized_eigh``,
    P = (I - X)/2 projects on the m >= nroots eigenvectors below sigma, and a
    Rayleigh-Ritz over range(P Omega) (m x m, on the host) gives the pairs;
    C = W^H U are the coefficients in the Davidson basis.
-4. Fallback: if the sign iteration has not converged (an eigenvalue sits at
-   sigma), or at the first Rayleigh-Ritz, the pencil is copied to the host
-   and solved by zhegvd exactly as the reference does.
+4. Fallback: at the first Rayleigh-Ritz, when fewer than nroots + 2 Ritz
+   values of the previous solve are known, or when the sign iteration has
+   not converged (an eigenvalue sits at sigma), the standardised A is
+   diagonalised densely on the device (``torch.linalg.eigh``, FP64).
+   ``spectral=False`` instead copies the pencil to the host and solves it by
+   zhegvd exactly as the reference does.
+
+The small host factorisations (Cholesky of the new overlap block, the
+eigenproblem of the sampled range) run on one BLAS thread: at these sizes
+(<= a few hundred) a multi-threaded BLAS next to torch's own OpenMP pool
+spends far longer in thread hand-off than in arithmetic.
 
 Eigenvalues and the S-orthonormality of the coefficients agree with zhegvd to
 rounding (tested against the host solve on every iteration of the small and
@@ -34,11 +42,21 @@ from __future__ import annotations
 import numpy as np
 import scipy.linalg as sla
 import torch
+from threadpoolctl import ThreadpoolController
 
 from . import _native as nat
 from . import device as dv
 
 C128 = torch.complex128
+_THREADPOOLS = None
+
+
+def _one_blas_thread():
+    """Context limiting the host BLAS/LAPACK to one thread."""
+    global _THREADPOOLS
+    if _THREADPOOLS is None:
+        _THREADPOOLS = ThreadpoolController()
+    return _THREADPOOLS.limit(limits=1, user_api="blas")
 
 
 def _scaled_ns_alphas(lower: float, tol: float = 1e-15, limit: int = 60) -> list[float]:
@@ -69,12 +87,14 @@ class DeviceRayleighRitz:
         # column-major matrices: tensor[c, r] holds element (r, c)
         self.H, self.S, self.W, self.A = z(), z(), z(), z()
         self.X, self.X2, self.Y = z(), z(), z()
-        self.bound = torch.zeros(1, dtype=torch.float64, device=device)
-        self.sig = torch.zeros(2, dtype=torch.float64, device=device)
+        # bound[0] and the column sums behind it; sig[0:2] and its partial sums
+        self.bound = torch.zeros(self.maxsub + 1, dtype=torch.float64, device=device)
+        self.sig = torch.zeros(self.maxsub + 2, dtype=torch.float64, device=device)
         self.standard = 0            # W and A are valid for [:standard]
         self.theta = None            # previous Ritz values (host, ascending)
         self.u_prev = None           # previous lowest eigenvectors of A (device, (k, maxsub))
-        self.stats = {"spectral": 0, "host": 0, "ns_iterations": 0, "max_check_error": 0.0}
+        self.stats = {"spectral": 0, "dense": 0, "host": 0, "ns_iterations": 0,
+                      "max_check_error": 0.0}
         self._rng = np.random.default_rng(1234)
 
     # -- views -------------------------------------------------------------
@@ -175,27 +195,30 @@ class DeviceRayleighRitz:
     def solve(self, active: int):
         """Lowest nroots pairs: (values host float64, coefficients device
         (nroots, active) column-major active x nroots)."""
-        k = self.nroots
-        use_spectral = (self.spectral and self.theta is not None and len(self.theta) > k
-                        and active > k + 8)
-        values = coef = None
-        if use_spectral:
-            self._standardize(active)
-            out = self._spectral(active)
-            if out is not None:
-                values, coef = out
-                self.stats["spectral"] += 1
-        if values is None:
-            values, coef, all_values = self._host(active)
-            self.theta = all_values
+        if not self.spectral:
+            values, coef, _ = self._host(active)
             self.stats["host"] += 1
-        if self.check and use_spectral and self.stats["spectral"]:
-            ref, _, _ = self._host(active, keep=False)
+            return values, coef
+        k = self.nroots
+        with _one_blas_thread():
+            self._standardize(active)
+            out = None
+            if self.theta is not None and len(self.theta) >= k + 2 and active > k + 8:
+                out = self._spectral(active)
+            if out is not None:
+                self.stats["spectral"] += 1
+            else:
+                out = self._dense(active)
+                self.stats["dense"] += 1
+        values, coef = out
+        if self.check:
+            ref, _, _ = self._host(active)
             err = float(np.max(np.abs(ref - values)))
             self.stats["max_check_error"] = max(self.stats["max_check_error"], err)
         return values, coef
 
-    def _host(self, active, keep=True):
+    def _host(self, active):
+        """zhegvd of the copied pencil on the host, as the reference does."""
         h = self.host_block(self.H, active)
         s = self.host_block(self.S, active)
         h = np.array(h, dtype=np.complex128, order="F")
@@ -207,9 +230,30 @@ class DeviceRayleighRitz:
             raise np.linalg.LinAlgError(f"zhegvd failed: {exc}") from exc
         k = self.nroots
         coef = torch.from_numpy(np.ascontiguousarray(v[:, :k].T)).to(self.device)
-        if keep:
-            self.u_prev = None  # the spectral path rebuilds it from its own solve
         return w[:k].copy(), coef, w.copy()
+
+    def _dense(self, n):
+        """All eigenpairs of the standardised A on the device; C = W^H U."""
+        k = self.nroots
+        # the column-major block is stored transposed: tensor = A^T = conj(A)
+        # (A Hermitian), whose eigenvectors are the conjugates of A's
+        w, vt = torch.linalg.eigh(self.A[:n, :n])
+        # (k, n): column-major n x k; conj_physical, because a lazy conj()
+        # view would hand the kernels the unconjugated memory
+        U = torch.conj_physical(vt[:, :k].T.contiguous())
+        coef = torch.empty((k, n), dtype=C128, device=self.device)
+        self._gemm(nat.OP_C, n, k, n, 1.0, self.block(self.W, 0, n, 0, n), U, 0.0, coef)
+        self._keep_vectors(U, n)
+        values = w.cpu().numpy().astype(np.float64)
+        self.theta = values
+        return values[:k].copy(), coef
+
+    def _keep_vectors(self, U, n):
+        k = self.nroots
+        if self.u_prev is None:
+            self.u_prev = torch.zeros((k, self.maxsub), dtype=C128, device=self.device)
+        self.u_prev.zero_()
+        self.u_prev[:, :n].copy_(U)
 
     def _spectral(self, n):
         k = self.nroots
@@ -217,33 +261,42 @@ class DeviceRayleighRitz:
         lib = nat.load()
         theta = self.theta
         hi = min(len(theta), k + self.extra)
-        # sigma in the widest gap among theta[k-1 .. hi-1]: at least k new
-        # eigenvalues lie below theta[k-1] <= sigma
-        gaps = np.diff(theta[k - 1:hi])
+        # sigma in the widest gap among theta[k-1+margin .. hi-1]: without a
+        # restart since the last solve the subspace only grew, so by Cauchy
+        # interlacing at least k + margin eigenvalues of the new A lie below
+        # sigma and the next solve again knows more than k Ritz values
+        margin = min(4, len(theta) - k - 1)
+        lo_i = k - 1 + margin
+        gaps = np.diff(theta[lo_i:hi])
         j = int(np.argmax(gaps))
-        lo_v, hi_v = theta[k - 1 + j], theta[k + j]
+        lo_v, hi_v = theta[lo_i + j], theta[lo_i + j + 1]
         sigma = 0.5 * (lo_v + hi_v)
         A = self.block(self.A, 0, n, 0, n)
         X, X2, Y = (self.block(m, 0, n, 0, n) for m in (self.X, self.X2, self.Y))
         nat.check(lib.qepw_shift_scale(A.data_ptr(), self.maxsub, n, sigma, X.data_ptr(),
                                        self.maxsub, self.bound.data_ptr(), st),
                   "qepw_shift_scale")
-        bound = float(self.bound.cpu()[0])
+        bound = float(self.bound[:1].cpu()[0])
         lower = 0.25 * (hi_v - lo_v) / bound
         alphas = _scaled_ns_alphas(lower) + [1.0, 1.0]
         converged = False
         for attempt in range(3):
-            for a in alphas:
+            for it, a in enumerate(alphas):
                 self._gemm(nat.OP_N, n, n, n, 1.0, X, X, 0.0, Y)
                 X2.copy_(X)
                 self._gemm(nat.OP_N, n, n, n, -0.5 * a ** 3, X, Y, 1.5 * a, X2)
-                nat.check(lib.qepw_hermitize(X2.data_ptr(), self.maxsub, n, st), "qepw_hermitize")
+                # X^3 is Hermitian up to rounding; re-symmetrising every
+                # fourth step (and before the check) keeps the drift at the
+                # rounding level
+                if it % 4 == 3 or it == len(alphas) - 1:
+                    nat.check(lib.qepw_hermitize(X2.data_ptr(), self.maxsub, n, st),
+                              "qepw_hermitize")
                 X, X2 = X2, X
                 self.stats["ns_iterations"] += 1
             self._gemm(nat.OP_N, n, n, n, 1.0, X, X, 0.0, Y)
             nat.check(lib.qepw_sign_check(X.data_ptr(), self.maxsub, Y.data_ptr(), self.maxsub, n,
                                           self.sig.data_ptr(), st), "qepw_sign_check")
-            res2, trace = (float(v) for v in self.sig.cpu().numpy())
+            res2, trace = (float(v) for v in self.sig[:2].cpu().numpy())
             if res2 < 1e-20 * n:
                 converged = True
                 break
@@ -253,7 +306,7 @@ class DeviceRayleighRitz:
         m_below = int(round(0.5 * (n - trace)))
         if m_below < k:
             return None
-        mp = min(n, m_below + 4)
+        mp = m_below
         omega = torch.zeros((mp, self.maxsub), dtype=C128, device=self.device)
         start = 0
         if self.u_prev is not None:
@@ -276,12 +329,14 @@ class DeviceRayleighRitz:
         b = gb[1].T
         g = 0.5 * (g + g.conj().T)
         b = 0.5 * (b + b.conj().T)
-        gw, gv = np.linalg.eigh(g)
-        # Q0 has rank m_below < mp: the remaining directions of G are the
-        # sign iteration's residue and rounding (relative size <= 1e-16 after
-        # the 1e-20 convergence test); sampled range directions are O(1)
-        keep = gw > 1e-12 * gw[-1]
-        t = gv[:, keep] / np.sqrt(gw[keep])[None, :]
+        # Q0 = P Omega has exactly rank m_below = mp (Omega: the previous
+        # eigenvectors, then Gaussian columns); its Gram matrix is factored by
+        # Cholesky, t = L^-H, so that (Q0 t)^H (Q0 t) = I
+        try:
+            lg = np.linalg.cholesky(g)
+            t = sla.solve_triangular(lg, np.eye(mp), lower=True).conj().T
+        except np.linalg.LinAlgError:
+            return None
         bt = t.conj().T @ b @ t
         bt = 0.5 * (bt + bt.conj().T)
         w, y = np.linalg.eigh(bt)
@@ -293,9 +348,6 @@ class DeviceRayleighRitz:
         self._gemm(nat.OP_N, n, k, mp, 1.0, Q0, z_dev, 0.0, U)
         coef = torch.empty((k, n), dtype=C128, device=self.device)
         self._gemm(nat.OP_C, n, k, n, 1.0, self.block(self.W, 0, n, 0, n), U, 0.0, coef)
-        if self.u_prev is None or self.u_prev.shape[0] != k:
-            self.u_prev = torch.zeros((k, self.maxsub), dtype=C128, device=self.device)
-        self.u_prev.zero_()
-        self.u_prev[:, :n].copy_(U)
+        self._keep_vectors(U, n)
         self.theta = np.asarray(w, dtype=np.float64)
         return w[:k].copy(), coef
