@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of oracle Davidson work for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 24-problem uneven batch leg")
+    ap.add_argument("--batch-workers", type=int, default=2)
     ap.add_argument("--breakdown", action="store_true",
                     help="print a per-label timer breakdown of one extra solve to stderr")
     ap.add_argument("--fft-probe", type=int, default=0, metavar="CALLS",
@@ -192,6 +195,73 @@ def run_form(form, ham, system, trial, kwargs, oracle_values, tol_ha, steps, war
     return record, events, results
 
 
+def batch_leg(args, device):
+    """The uneven batched workload: 3 cells (54^3/64^3/72^3 grids) x 8
+    seeded k-points = 24 problems in the mixed form, solved together by
+    ``davidson_batch`` and, for comparison, one by one."""
+    import numpy as np
+    import torch
+
+    from qe_python_amd.batch import BatchProblem, davidson_batch
+    from qe_python_amd.diagonalization import PackedProjectorTerm, PlaneWaveHamiltonian, davidson
+    from qe_python_amd.synthetic import batch_kpoints, make_system
+
+    golden = np.load(ROOT / "tests" / "golden" / "batch_davidson.npz")
+    tol_ha = 0.5 * args.tol_ry
+    kwargs = dict(tolerance=args.tol_factor * tol_ha, residual_factor=None,
+                  subspace_multiplier=4, max_iterations=40, precision="mixed")
+    problems, names, npws, roots = [], [], [], []
+    for cell in ("cell54", "cell64", "cell72"):
+        for k, kp in enumerate(batch_kpoints(cell)):
+            s = make_system(cell, kpoint=kp)
+            beta, coup, diag = s.packed_term()
+            ham = PlaneWaveHamiltonian(s.basis, s.potential_g,
+                                       (PackedProjectorTerm(beta, coup, diag),),
+                                       real_potential=s.real_potential, device=device)
+            trial = torch.from_numpy(s.trial).to(device)
+            problems.append(BatchProblem(ham.apply, ham.diagonal, s.nroots, trial,
+                                         ham.apply_into))
+            names.append(f"{cell}_k{k}")
+            npws.append(s.npw)
+            roots.append(s.nroots)
+    print(f"batch: {len(problems)} problems built", file=sys.stderr, flush=True)
+    davidson_batch(problems, workers=args.batch_workers, device_output=True, **kwargs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    results = davidson_batch(problems, workers=args.batch_workers, device_output=True, **kwargs)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for p in problems:
+        davidson(p.operator, p.diagonal, p.number_of_roots, initial_vectors=p.initial_vectors,
+                 operator_into=p.operator_into, device_output=True, **kwargs)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    checked = [n for n in names if n in golden.files]
+    errors = {n: float(np.max(np.abs(r.eigenvalues - golden[n])))
+              for n, r in zip(names, results) if n in golden.files}
+    valid = all(r.converged for r in results) and max(errors.values()) <= tol_ha
+    print(f"batch: {1e3 * t_batch:.0f} ms together, {1e3 * t_seq:.0f} ms one by one, "
+          f"valid {valid}", file=sys.stderr, flush=True)
+    return {
+        "workload": "3 cells (a=15.39/17.95/20.52 bohr, grids 54^3/64^3/72^3, 27/42/64 atoms) "
+                    "x 8 seeded k-points = 24 problems",
+        "form": "mixed",
+        "value": len(problems) / t_batch if valid else 0.0,
+        "unit": "problems/s",
+        "ms_batch": 1e3 * t_batch,
+        "ms_one_by_one": 1e3 * t_seq,
+        "speedup_over_one_by_one": t_seq / t_batch,
+        "workers": args.batch_workers,
+        "npw_range": [min(npws), max(npws)],
+        "roots": sorted(set(roots)),
+        "iterations": [r.iterations for r in results],
+        "checked_against_oracle": checked,
+        "max_eigenvalue_error_ha": max(errors.values()),
+        "valid": valid,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -309,6 +379,8 @@ def main():
                 print(f"  {k:14s} {v.wall_seconds * 1e3:9.1f} ms  {v.calls:5d} calls",
                       file=sys.stderr)
             ham.timers = None
+    if not args.no_batch:
+        line["batch"] = batch_leg(args, device)
     if not args.no_cpu:
         happ = head["hamiltonian_applications"][-1]
         line["cpu_baseline"] = cpu_baseline(system, kwargs, happ, args.cpu_budget)

@@ -174,6 +174,17 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
   const cplx<T> zero = mk<T>(T(0), T(0));
   cplx<T>* w = work + b * band_stride + (int64_t)z * nst;
 
+  // per-lane tables, the same for every line of the plane: twiddles, and
+  // the active-column slots of this lane's row inputs j = t + L s and row
+  // outputs k = t + M s (-1: inactive x column)
+  TeamTwiddles<T, N> twr;
+  twr.load(t, tw);
+  int cin[G::M], cout[G::L];
+#pragma unroll
+  for (int s = 0; s < G::M; ++s) cin[s] = t < G::L ? colmap[t + G::L * s] : -1;
+#pragma unroll
+  for (int s = 0; s < G::L; ++s) cout[s] = t < G::M ? colmap[t + G::M * s] : -1;
+
   for (int i = tid; i < nq; i += kTeamThreads) Q[i] = zero;
   __syncthreads();
   for (int s = tid; s < nst; s += kTeamThreads) Q[stick_q[s]] = w[s];
@@ -183,8 +194,8 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
   if (in_team) {
     for (int c = team; c < nxc; c += G::teams) {
       cplx<T>* col = Q + c * N;
-      team_fft<T, N, +1>(t, S, tw, [&](int j) { return col[j]; },
-                         [&](int k, cplx<T> v) { col[k] = v; });
+      team_fft<T, N, +1>(t, S, twr, [&](int j, int) { return col[j]; },
+                         [&](int k, int, cplx<T> v) { col[k] = v; });
     }
   }
   __syncthreads();
@@ -194,22 +205,21 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
   if (in_team) {
     const T* vz = V + (int64_t)z * N * N;
     for (int y = team; y < N; y += G::teams) {
-      const T* vrow = vz + y * N;
+      // V(r) of this lane's outputs, fetched before the FFT needs them
+      T vv[G::L];
+#pragma unroll
+      for (int s = 0; s < G::L; ++s) vv[s] = t < G::M ? vz[y * N + t + G::M * s] : T(0);
       team_fft<T, N, +1>(
-          t, S, tw,
-          [&](int j) {
-            const int c = colmap[j];
-            return c >= 0 ? Q[c * N + y] : zero;
-          },
-          [&](int k, cplx<T> v) { S[k] = cscale(v, vrow[k]); });
+          t, S, twr,
+          [&](int, int s) { return cin[s] >= 0 ? Q[cin[s] * N + y] : zero; },
+          [&](int k, int s, cplx<T> v) { S[k] = cscale(v, vv[s]); });
       // S now holds V psi of this row in natural order (written after the
       // team's last exchange read, read back before any exchange write)
       wave_lds_sync();
       team_fft<T, N, -1>(
-          t, S, tw, [&](int j) { return S[j]; },
-          [&](int k, cplx<T> v) {
-            const int c = colmap[k];
-            if (c >= 0) Q[c * N + y] = v;
+          t, S, twr, [&](int j, int) { return S[j]; },
+          [&](int, int s, cplx<T> v) {
+            if (cout[s] >= 0) Q[cout[s] * N + y] = v;
           });
     }
   }
@@ -219,8 +229,8 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
   if (in_team) {
     for (int c = team; c < nxc; c += G::teams) {
       cplx<T>* col = Q + c * N;
-      team_fft<T, N, -1>(t, S, tw, [&](int j) { return col[j]; },
-                         [&](int k, cplx<T> v) { col[k] = v; });
+      team_fft<T, N, -1>(t, S, twr, [&](int j, int) { return col[j]; },
+                         [&](int k, int, cplx<T> v) { col[k] = v; });
     }
   }
   __syncthreads();
@@ -401,6 +411,11 @@ __global__ __launch_bounds__(kTeamThreads) void k_density_team(
   const int b1 = min(nb, b0 + bands_per_slice);
   const int nq = nxc * N;
   const cplx<T> zero = mk<T>(T(0), T(0));
+  TeamTwiddles<T, N> twr;
+  twr.load(t, tw);
+  int cin[G::M];
+#pragma unroll
+  for (int s = 0; s < G::M; ++s) cin[s] = t < G::L ? colmap[t + G::L * s] : -1;
   for (int i = tid; i < N * N; i += kTeamThreads) D[i] = 0.0;
   for (int b = b0; b < b1; ++b) {
     const cplx<T>* w = work + (int64_t)b * band_stride + (int64_t)z * nst;
@@ -412,8 +427,8 @@ __global__ __launch_bounds__(kTeamThreads) void k_density_team(
     if (in_team) {
       for (int c = team; c < nxc; c += G::teams) {
         cplx<T>* col = Q + c * N;
-        team_fft<T, N, +1>(t, S, tw, [&](int j) { return col[j]; },
-                           [&](int k, cplx<T> v) { col[k] = v; });
+        team_fft<T, N, +1>(t, S, twr, [&](int j, int) { return col[j]; },
+                           [&](int k, int, cplx<T> v) { col[k] = v; });
       }
     }
     __syncthreads();
@@ -421,12 +436,9 @@ __global__ __launch_bounds__(kTeamThreads) void k_density_team(
       for (int y = team; y < N; y += G::teams) {
         double* drow = D + y * N;
         team_fft<T, N, +1>(
-            t, S, tw,
-            [&](int j) {
-              const int c = colmap[j];
-              return c >= 0 ? Q[c * N + y] : zero;
-            },
-            [&](int k, cplx<T> v) {
+            t, S, twr,
+            [&](int, int s) { return cin[s] >= 0 ? Q[cin[s] * N + y] : zero; },
+            [&](int k, int, cplx<T> v) {
               drow[k] += wb * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
             });
       }

@@ -6,9 +6,11 @@
 //   op C : re += Ar Br + Ai Bi ;  im += Ar Bi - Ai Br
 // (op(B) = B^H is conjugated as it is staged into LDS).
 // using v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (FP32, FP32
-// accumulation).  A 256-thread workgroup owns a 64x64 output tile; each of
-// its four waves owns a 32x32 quarter (2x2 MFMA blocks of 16x16) and walks K
-// in steps of 16 staged through LDS.  Tall-skinny A^H B products (K = npw up
+// accumulation).  A 256-thread workgroup owns a TILE x TILE output tile
+// (TILE = 64, or 32 for small outputs such as the reduced problem's square
+// products, where 64x64 tiles leave most CUs idle); each of its four waves
+// owns a quarter (2x2 or 1x1 MFMA blocks of 16x16) and walks K in steps of
+// 16 staged through LDS.  Tall-skinny A^H B products (K = npw up
 // to ~24k, M,N <= 512) are split over K: every split writes its partial tile
 // to the workspace and a second kernel sums the splits in fixed order, so
 // results are bitwise reproducible run to run.
@@ -40,52 +42,54 @@ template <> struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-constexpr int kTile = 64;
+constexpr int kTile = 64;     // largest tile (grid-size checks)
 constexpr int kBK = 16;
 constexpr int kGemmThreads = 256;
 constexpr int kPad = 1;
 
-template <typename T, bool CONJ_A, bool CONJ_B>
+template <typename T, bool CONJ_A, bool CONJ_B, int TILE>
 __global__ __launch_bounds__(kGemmThreads) void k_cgemm(
     int64_t M, int64_t N, int64_t K, int64_t kchunk, const cplx<T>* __restrict__ A,
     int64_t lda, const cplx<T>* __restrict__ B, int64_t ldb, cplx<T>* __restrict__ C,
     int64_t ldc, cplx<T>* __restrict__ partial, T alpha_r, T alpha_i, T beta_r,
     T beta_i) {
   using Acc = typename Mfma<T>::acc;
-  __shared__ cplx<T> As[kBK][kTile + kPad];
-  __shared__ cplx<T> Bs[kBK][kTile + kPad];
+  constexpr int WT = TILE / 2;    // wave tile
+  constexpr int NB = WT / 16;     // MFMA blocks per wave and dimension
+  __shared__ cplx<T> As[kBK][TILE + kPad];
+  __shared__ cplx<T> Bs[kBK][TILE + kPad];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = (wave & 1) * 32;
-  const int wn = (wave >> 1) * 32;
-  const int64_t m0 = (int64_t)blockIdx.x * kTile;
-  const int64_t n0 = (int64_t)blockIdx.y * kTile;
+  const int wm = (wave & 1) * WT;
+  const int wn = (wave >> 1) * WT;
+  const int64_t m0 = (int64_t)blockIdx.x * TILE;
+  const int64_t n0 = (int64_t)blockIdx.y * TILE;
   const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
   const int64_t kend = (kbeg + kchunk) < K ? (kbeg + kchunk) : K;
   const cplx<T> zero = mk<T>(T(0), T(0));
 
-  Acc re[2][2], im[2][2];
+  Acc re[NB][NB], im[NB][NB];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < NB; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < NB; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) { re[i][j][v] = T(0); im[i][j][v] = T(0); }
 
   for (int64_t k0 = kbeg; k0 < kend; k0 += kBK) {
 #pragma unroll
-    for (int e = tid; e < kBK * kTile; e += kGemmThreads) {
+    for (int e = tid; e < kBK * TILE; e += kGemmThreads) {
       int kk, mm;
-      if (CONJ_A) { kk = e & (kBK - 1); mm = e >> 4; }
-      else { mm = e & (kTile - 1); kk = e >> 6; }
+      if (CONJ_A) { kk = e & (kBK - 1); mm = e / kBK; }
+      else { mm = e & (TILE - 1); kk = e / TILE; }
       const int64_t k = k0 + kk, m = m0 + mm;
       cplx<T> v = zero;
       if (k < kend && m < M) v = CONJ_A ? A[k + m * lda] : A[m + k * lda];
       As[kk][mm] = v;
       int kb, nb;
-      if (CONJ_B) { nb = e & (kTile - 1); kb = e >> 6; }
-      else { kb = e & (kBK - 1); nb = e >> 4; }
+      if (CONJ_B) { nb = e & (TILE - 1); kb = e / TILE; }
+      else { kb = e & (kBK - 1); nb = e / kBK; }
       const int64_t kq = k0 + kb, n = n0 + nb;
       cplx<T> w = zero;
       if (kq < kend && n < N) {
@@ -102,17 +106,17 @@ __global__ __launch_bounds__(kGemmThreads) void k_cgemm(
 #pragma unroll
     for (int ks = 0; ks < kBK; ks += 4) {
       const int kk = ks + (lane >> 4);
-      cplx<T> a[2], b[2];
+      cplx<T> a[NB], b[NB];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[kk][wm + 16 * i + (lane & 15)];
+      for (int i = 0; i < NB; ++i) a[i] = As[kk][wm + 16 * i + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Bs[kk][wn + 16 * j + (lane & 15)];
+      for (int j = 0; j < NB; ++j) b[j] = Bs[kk][wn + 16 * j + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < NB; ++i) {
         const T ai_re = CONJ_A ? a[i].y : -a[i].y;
         const T ai_im = CONJ_A ? -a[i].y : a[i].y;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < NB; ++j) {
           re[i][j] = Mfma<T>::op(a[i].x, b[j].x, re[i][j]);
           re[i][j] = Mfma<T>::op(ai_re, b[j].y, re[i][j]);
           im[i][j] = Mfma<T>::op(a[i].x, b[j].y, im[i][j]);
@@ -125,9 +129,9 @@ __global__ __launch_bounds__(kGemmThreads) void k_cgemm(
 
   const bool direct = gridDim.z == 1;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < NB; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < NB; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t row = m0 + wm + 16 * i + Mfma<T>::row(lane, v);
@@ -177,10 +181,15 @@ __global__ __launch_bounds__(256) void k_reduce_splits(
 struct SplitPlan {
   int64_t splits;
   int64_t kchunk;
+  int tile;
 };
 
 static SplitPlan split_plan(int op_a, int64_t m, int64_t n, int64_t k) {
-  const int64_t tiles = ((m + kTile - 1) / kTile) * ((n + kTile - 1) / kTile);
+  const int64_t tiles64 = ((m + 63) / 64) * ((n + 63) / 64);
+  // outputs of fewer than 256 64x64 tiles take 32x32 tiles, except the tall
+  // A^H B products over npw, whose parallelism comes from splitting K
+  const int tile = (tiles64 < 256 && !(op_a == QEPW_OP_C && k > 2048)) ? 32 : 64;
+  const int64_t tiles = ((m + tile - 1) / tile) * ((n + tile - 1) / tile);
   int64_t splits = 1;
   // split K when the output tiles alone cannot fill the 256 CUs: the tall
   // A^H B products over npw and the small square products of the reduced
@@ -196,7 +205,7 @@ static SplitPlan split_plan(int op_a, int64_t m, int64_t n, int64_t k) {
   if (kchunk < kBK) kchunk = kBK;
   splits = (k + kchunk - 1) / kchunk;
   if (splits < 1) splits = 1;
-  return {splits, kchunk};
+  return {splits, kchunk, tile};
 }
 
 size_t gemm_workspace_bytes(int precision, int op, int64_t m, int64_t n, int64_t k) {
@@ -214,22 +223,28 @@ static int gemm_t(int op, int64_t m, int64_t n, int64_t k, const double* alpha,
   if (sp.splits > 65535) return QEPW_ERR_UNSUPPORTED;
   const size_t need = sp.splits > 1 ? (size_t)sp.splits * m * n * sizeof(cplx<T>) : 0;
   if (ws_bytes < need) return QEPW_ERR_WORKSPACE;
-  const dim3 grid((unsigned)((m + kTile - 1) / kTile), (unsigned)((n + kTile - 1) / kTile),
-                  (unsigned)sp.splits);
+  const dim3 grid((unsigned)((m + sp.tile - 1) / sp.tile),
+                  (unsigned)((n + sp.tile - 1) / sp.tile), (unsigned)sp.splits);
   const T ar = (T)alpha[0], ai = (T)alpha[1], br = (T)beta[0], bi = (T)beta[1];
   cplx<T>* part = static_cast<cplx<T>*>(ws);
   const cplx<T>* pa = static_cast<const cplx<T>*>(a);
   const cplx<T>* pb = static_cast<const cplx<T>*>(b);
   cplx<T>* pc = static_cast<cplx<T>*>(c);
-#define QEPW_GEMM_LAUNCH(CA, CB)                                                            \
-  hipLaunchKernelGGL((k_cgemm<T, CA, CB>), grid, dim3(kGemmThreads), 0, stream, m, n, k, \
-                     sp.kchunk, pa, lda, pb, ldb, pc, ldc, part, ar, ai, br, bi)
+#define QEPW_GEMM_TILE(CA, CB, TL)                                                         \
+  hipLaunchKernelGGL((k_cgemm<T, CA, CB, TL>), grid, dim3(kGemmThreads), 0, stream, m, n, \
+                     k, sp.kchunk, pa, lda, pb, ldb, pc, ldc, part, ar, ai, br, bi)
+#define QEPW_GEMM_LAUNCH(CA, CB)                   \
+  do {                                             \
+    if (sp.tile == 32) QEPW_GEMM_TILE(CA, CB, 32); \
+    else QEPW_GEMM_TILE(CA, CB, 64);               \
+  } while (0)
   const bool ca = (op & QEPW_OP_C) != 0, cb = (op & QEPW_OP_BC) != 0;
   if (ca && cb) QEPW_GEMM_LAUNCH(true, true);
   else if (ca) QEPW_GEMM_LAUNCH(true, false);
   else if (cb) QEPW_GEMM_LAUNCH(false, true);
   else QEPW_GEMM_LAUNCH(false, false);
 #undef QEPW_GEMM_LAUNCH
+#undef QEPW_GEMM_TILE
   if (sp.splits > 1) {
     const int64_t total = m * n;
     hipLaunchKernelGGL(k_reduce_splits<T>, dim3((unsigned)((total + 255) / 256)), dim3(256),

@@ -142,27 +142,41 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Per-lane twiddles of a team FFT: w[k1] = w_N^(t k1), k1 < M, for the
+// lane's index t < L in its team.  They are the same for every line of a
+// plane, so a kernel loads them into registers once.
+template <typename T, int N> struct TeamTwiddles {
+  static constexpr int L = team_L(N);
+  static constexpr int M = N / L;
+  cplx<T> w[M];
+  __device__ __forceinline__ void load(int t, const cplx<T>* __restrict__ tw) {
+#pragma unroll
+    for (int k1 = 0; k1 < M; ++k1) w[k1] = t < L ? tw[t * k1] : mk<T>(T(1), T(0));
+  }
+};
+
 // One N-point line by the M lanes of a team; ``t`` is the lane's index in
-// the team; ``scratch`` holds N elements private to the team; ``tw`` is the
-// w_N^k = exp(-2 pi i k / N) table.  load(j) gives input j, store(k, v)
-// receives output k (each exactly once, by lane k % M).
+// the team; ``scratch`` holds N elements private to the team; ``tw`` holds
+// the lane's twiddles.  load(j, s) gives input j = t + L s (s < M, lanes
+// t < L); store(k, s, v) receives output k = t + M s (s < L, lanes t < M),
+// each exactly once; the slot index s lets callers keep per-lane tables in
+// registers.
 template <typename T, int N, int SIGN, class Load, class Store>
-__device__ __forceinline__ void team_fft(int t, cplx<T>* scratch,
-                                         const cplx<T>* __restrict__ tw, Load load,
-                                         Store store) {
+__device__ __forceinline__ void team_fft(int t, cplx<T>* scratch, const TeamTwiddles<T, N>& tw,
+                                         Load load, Store store) {
   constexpr int L = team_L(N);
   constexpr int M = N / L;
   static_assert(L >= 1 && M >= L && M <= 16, "unsupported team FFT length");
   if (t < L) {
     cplx<T> a[M];
 #pragma unroll
-    for (int j2 = 0; j2 < M; ++j2) a[j2] = load(t + L * j2);
+    for (int j2 = 0; j2 < M; ++j2) a[j2] = load(t + L * j2, j2);
     RegDft<T, M, SIGN>::run(a);
 #pragma unroll
     for (int k1 = 0; k1 < M; ++k1) {
       cplx<T> v = a[k1];
       if (k1 > 0 && t > 0) {
-        cplx<T> w = tw[t * k1];
+        cplx<T> w = tw.w[k1];
         if (SIGN > 0) w.y = -w.y;
         v = cmul(v, w);
       }
@@ -176,7 +190,7 @@ __device__ __forceinline__ void team_fft(int t, cplx<T>* scratch,
     for (int j1 = 0; j1 < L; ++j1) b[j1] = scratch[j1 * M + t];
     RegDft<T, L, SIGN>::run(b);
 #pragma unroll
-    for (int k2 = 0; k2 < L; ++k2) store(t + M * k2, b[k2]);
+    for (int k2 = 0; k2 < L; ++k2) store(t + M * k2, k2, b[k2]);
   }
   wave_lds_sync();
 }

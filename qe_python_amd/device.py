@@ -197,8 +197,9 @@ class DeviceFFTPlan:
 
     def __del__(self):
         handle = getattr(self, "handle", None)
-        if handle is not None and nat._LIB is not None:
-            nat._LIB.qepw_plan_destroy(handle)
+        lib = getattr(nat, "_LIB", None) if nat is not None else None
+        if handle is not None and lib is not None:  # (module globals are gone at exit)
+            lib.qepw_plan_destroy(handle)
             self.handle = None
 
     def workspace_bytes(self, dtype: torch.dtype, nb: int) -> int:

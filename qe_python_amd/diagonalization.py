@@ -473,6 +473,34 @@ class _HostOperator:
         dst.copy_(torch.from_numpy(np.ascontiguousarray(out.T)).to(dst.dtype))
 
 
+def _cholesky_orthonormalize(block: torch.Tensor, npw: int, ws, condition: float = 1.0e6):
+    """Cholesky QR of a block of complex64 corrections: Gram = L L^H on the
+    host (one thread), block L^-H on the device.  None when the Gram matrix
+    is not safely positive definite at FP32 precision (its Cholesky fails or
+    the squared condition estimate max(diag L)^2 / min(diag L)^2 exceeds
+    ``condition``); the caller then drops dependent directions with the
+    canonical orthonormalisation."""
+    n = int(block.shape[0])
+    device = block.device
+    gram_t = torch.empty((n, n), dtype=block.dtype, device=device)
+    dv.gemm(nat.OP_C, n, n, npw, 1.0, block, block, 0.0, gram_t, ws)
+    gram = gram_t.cpu().numpy().T.astype(np.complex128)
+    gram = 0.5 * (gram + gram.conj().T)
+    with _one_blas_thread():
+        try:
+            chol = np.linalg.cholesky(gram)
+        except np.linalg.LinAlgError:
+            return None
+        d = np.abs(np.diag(chol))
+        if d.min() <= 0.0 or (d.max() / d.min()) ** 2 > condition:
+            return None
+        inv = sla.solve_triangular(chol, np.eye(n), lower=True).conj().T   # L^-H
+    t_dev = torch.from_numpy(np.ascontiguousarray(inv.T)).to(device=device, dtype=block.dtype)
+    out = torch.empty_like(block)
+    dv.gemm(nat.OP_N, npw, n, n, 1.0, block, t_dev, 0.0, out, ws)
+    return out
+
+
 def _orthonormalize_device(block: torch.Tensor, npw: int, ws, threshold: float = 1.0e-12):
     """Canonical orthonormalisation (``diagonalization.py:450-472``, serial):
     Gram V^H V on the device, its small Hermitian eigensolve on the host,
@@ -812,7 +840,9 @@ def _davidson_phase(ctx, start, dtype, energy_bounds, residual_bounds, residual_
                             0.0, proj, ws)
                     dv.gemm(nat.OP_N, npw, len(cols), active, -1.0, V[:active], proj, 1.0,
                             corrections, ws)
-                block = _orthonormalize_device(corrections, npw, ws, threshold=1.0e-4)
+                block = _cholesky_orthonormalize(corrections, npw, ws)
+                if block is None:
+                    block = _orthonormalize_device(corrections, npw, ws, threshold=1.0e-4)
             if not block.shape[0]:
                 break
             old = active

@@ -94,7 +94,8 @@ class DeviceRayleighRitz:
         self.theta = None            # previous Ritz values (host, ascending)
         self.u_prev = None           # previous lowest eigenvectors of A (device, (k, maxsub))
         self.stats = {"spectral": 0, "dense": 0, "host": 0, "ns_iterations": 0,
-                      "max_check_error": 0.0}
+                      "max_check_error": 0.0, "skipped": 0, "ns_unconverged": 0,
+                      "too_few_below": 0, "gram_not_pd": 0}
         self._rng = np.random.default_rng(1234)
 
     # -- views -------------------------------------------------------------
@@ -205,6 +206,8 @@ class DeviceRayleighRitz:
             out = None
             if self.theta is not None and len(self.theta) >= k + 2 and active > k + 8:
                 out = self._spectral(active)
+            else:
+                self.stats["skipped"] += 1
             if out is not None:
                 self.stats["spectral"] += 1
             else:
@@ -302,9 +305,11 @@ class DeviceRayleighRitz:
                 break
             alphas = [1.0] * 12
         if not converged:
+            self.stats["ns_unconverged"] += 1
             return None
         m_below = int(round(0.5 * (n - trace)))
         if m_below < k:
+            self.stats["too_few_below"] += 1
             return None
         mp = m_below
         omega = torch.zeros((mp, self.maxsub), dtype=C128, device=self.device)
@@ -336,6 +341,7 @@ class DeviceRayleighRitz:
             lg = np.linalg.cholesky(g)
             t = sla.solve_triangular(lg, np.eye(mp), lower=True).conj().T
         except np.linalg.LinAlgError:
+            self.stats["gram_not_pd"] += 1
             return None
         bt = t.conj().T @ b @ t
         bt = 0.5 * (bt + bt.conj().T)

@@ -48,7 +48,7 @@ def main():
     print("lowest dense eigenvalues (Ry):", 2 * evals[:10])
 
 
-if __name__ == "__main__" and "--medium" not in sys.argv:
+if __name__ == "__main__" and len(sys.argv) == 1:
     main()
 
 
@@ -72,3 +72,34 @@ def medium():
 
 if __name__ == "__main__" and "--medium" in sys.argv:
     medium()
+
+
+BATCH_PROBLEMS = [(cell, k) for cell in ("cell54", "cell64", "cell72") for k in (0, 1)]
+
+
+def batch():
+    """batch_davidson.npz: oracle Davidson (tolerance 0.5e-10 Ha,
+    residual_factor=None, up to 200 iterations) of the first two seeded k-points of each cell of
+    the uneven batched workload, keys ``<cell>_k<i>``; a few minutes on 8
+    cores."""
+    from oracle.davidson import davidson
+    from qe_python_amd.synthetic import batch_kpoints
+    out = {}
+    for cell, k in BATCH_PROBLEMS:
+        s = make_system(cell, kpoint=batch_kpoints(cell)[k])
+        beta, coup, diag = s.packed_term()
+        term = SimpleNamespace(beta=beta, coupling=coup, diagonal=diag)
+        ham = OracleHamiltonian(s.basis.indices, s.basis.kinetic, s.shape, s.real_potential,
+                                (term,), potential_average=float(np.real(s.potential_g.flat[0])))
+        res = davidson(ham.apply, ham.diagonal, s.nroots, initial_vectors=s.trial,
+                       tolerance=0.5e-10, residual_factor=None, operator_into=ham.apply_into,
+                       max_iterations=200)
+        assert res.converged, (cell, k)
+        out[f"{cell}_k{k}"] = res.eigenvalues
+        out[f"{cell}_k{k}_npw"] = np.asarray(s.npw)
+        print(cell, k, s.npw, res.iterations, res.converged, flush=True)
+    np.savez_compressed(Path(__file__).parent / "batch_davidson.npz", **out)
+
+
+if __name__ == "__main__" and "--batch" in sys.argv:
+    batch()

@@ -35,7 +35,7 @@ def rand(rng, *shape):
 @pytest.mark.parametrize("op", ["N", "C", "NB", "CB"])
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (5, 7, 3), (64, 64, 16), (65, 130, 37),
                                    (256, 128, 23847), (17, 300, 4000), (2945, 8, 32),
-                                   (3, 2, 0)])
+                                   (384, 384, 384), (500, 130, 257), (3, 2, 0)])
 def test_gemm_matches_numpy(prec, op, m, n, k):
     from qe_python_amd import _native as nat
     from qe_python_amd import device as dv
