@@ -260,6 +260,11 @@ int qepw_projector_forces(int precision, const void* dO, int64_t lddo, int64_t d
  * qepw_sign_check: out[0] = ||X2 - I||_F^2, out[1] = Re trace(X) (device);
  *   ``out`` holds n + 2 doubles (out[2 + j] = per-column partial sums).
  * qepw_set_diagonal: A[:n, :n] = diag(d), identity when d is NULL.
+ * qepw_sign_iteration: the scaled Newton-Schulz steps
+ *   X <- (a/2) X (3 I - a^2 X^2), one per entry of alphas[0:nalpha], on the
+ *   n x n complex128 X held in x (x2 and y: scratch of the same shape and
+ *   leading dimension); the result is left in x2 when *in_x2 is set, else
+ *   in x.  Workspace: qepw_sign_iteration_workspace(n) bytes.
  */
 int qepw_hermitian_complete(void* a, int64_t lda, int first, int active, void* stream);
 int qepw_shift_scale(const void* a, int64_t lda, int n, double sigma, void* x, int64_t ldx,
@@ -268,6 +273,10 @@ int qepw_hermitize(void* x, int64_t ldx, int n, void* stream);
 int qepw_sign_check(const void* x, int64_t ldx, const void* x2, int64_t ld2, int n,
                     double* out, void* stream);
 int qepw_set_diagonal(void* a, int64_t lda, int n, const double* d, void* stream);
+size_t qepw_sign_iteration_workspace(int n);
+int qepw_sign_iteration(void* x, void* x2, void* y, int64_t ld, int n, const double* alphas,
+                        int nalpha, int* in_x2, void* workspace, size_t workspace_bytes,
+                        void* stream);
 
 #ifdef __cplusplus
 }

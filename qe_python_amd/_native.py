@@ -76,6 +76,10 @@ SIGNATURES = [
     ("qepw_hermitize", c_int, [c_void_p, c_int64, c_int, c_void_p]),
     ("qepw_sign_check", c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int, c_void_p, c_void_p]),
     ("qepw_set_diagonal", c_int, [c_void_p, c_int64, c_int, c_void_p, c_void_p]),
+    ("qepw_sign_iteration_workspace", c_size_t, [c_int]),
+    ("qepw_sign_iteration", c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int,
+                                    POINTER(c_double), c_int, POINTER(c_int), c_void_p,
+                                    c_size_t, c_void_p]),
 ]
 
 _LIB: ctypes.CDLL | None = None

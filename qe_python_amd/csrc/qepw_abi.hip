@@ -492,6 +492,25 @@ int qepw_sign_check(const void* x, int64_t ldx, const void* x2, int64_t ld2, int
                 "qepw_sign_check");
 }
 
+size_t qepw_sign_iteration_workspace(int n) {
+  return n > 0 ? qepw::sign_iteration_workspace(n) : 0;
+}
+
+int qepw_sign_iteration(void* x, void* x2, void* y, int64_t ld, int n, const double* alphas,
+                        int nalpha, int* in_x2, void* workspace, size_t workspace_bytes,
+                        void* stream) {
+  if (n <= 0 || nalpha < 0 || !x || !x2 || !y || !in_x2 || ld < n || (nalpha > 0 && !alphas))
+    return fail(QEPW_ERR_ARG, "qepw_sign_iteration: bad argument");
+  if (x == x2 || x == y || x2 == y)
+    return fail(QEPW_ERR_ARG, "qepw_sign_iteration: x, x2 and y must be distinct");
+  if (workspace_bytes < qepw::sign_iteration_workspace(n))
+    return fail(QEPW_ERR_WORKSPACE, "qepw_sign_iteration: workspace too small");
+  *in_x2 = 0;
+  return finish(qepw::sign_iteration(x, x2, y, ld, n, alphas, nalpha, in_x2, workspace,
+                                     workspace_bytes, static_cast<hipStream_t>(stream)),
+                "qepw_sign_iteration");
+}
+
 int qepw_set_diagonal(void* a, int64_t lda, int n, const double* d, void* stream) {
   if (n < 0 || (n > 0 && (!a || lda < n)))
     return fail(QEPW_ERR_ARG, "qepw_set_diagonal: bad argument");

@@ -68,5 +68,8 @@ int hermitize(void* x, int64_t ldx, int n, hipStream_t stream);
 int sign_check(const void* x, int64_t ldx, const void* x2, int64_t ld2, int n, double* out,
                hipStream_t stream);
 int set_diagonal(void* a, int64_t lda, int n, const double* d, hipStream_t stream);
+size_t sign_iteration_workspace(int n);
+int sign_iteration(void* x, void* x2, void* y, int64_t ld, int n, const double* alphas,
+                   int nalpha, int* in_x2, void* ws, size_t ws_bytes, hipStream_t stream);
 
 }  // namespace qepw

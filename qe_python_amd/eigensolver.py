@@ -39,6 +39,8 @@ medium workloads).
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import scipy.linalg as sla
 import torch
@@ -48,6 +50,8 @@ from . import _native as nat
 from . import device as dv
 
 C128 = torch.complex128
+# dense fallback on the host up to this size (see DeviceRayleighRitz._dense)
+HOST_DENSE_MAX = 224
 _THREADPOOLS = None
 
 
@@ -92,6 +96,7 @@ class DeviceRayleighRitz:
         self.sig = torch.zeros(self.maxsub + 2, dtype=torch.float64, device=device)
         self.standard = 0            # W and A are valid for [:standard]
         self.theta = None            # previous Ritz values (host, ascending)
+        self.sigma = None            # shift of the previous spectral solve
         self.u_prev = None           # previous lowest eigenvectors of A (device, (k, maxsub))
         self.stats = {"spectral": 0, "dense": 0, "host": 0, "ns_iterations": 0,
                       "max_check_error": 0.0, "skipped": 0, "ns_unconverged": 0,
@@ -140,6 +145,7 @@ class DeviceRayleighRitz:
             nat.check(lib.qepw_set_diagonal(mat.data_ptr(), self.maxsub, k, d, st),
                       "qepw_set_diagonal")
         self.standard = k
+        self.sigma = None
         if self.u_prev is not None:
             # the Ritz vectors are the new basis: the eigenvectors of A = diag
             self.u_prev.zero_()
@@ -236,19 +242,29 @@ class DeviceRayleighRitz:
         return w[:k].copy(), coef, w.copy()
 
     def _dense(self, n):
-        """All eigenpairs of the standardised A on the device; C = W^H U."""
+        """All eigenpairs of the standardised A; C = W^H U.  Up to
+        ``HOST_DENSE_MAX`` rows the matrix is diagonalised on the host (one
+        BLAS thread, ~2 ms at 150), above on the device (rocSOLVER through
+        ``torch.linalg.eigh``, ~8-15 ms at 256-512, flat in n)."""
         k = self.nroots
-        # the column-major block is stored transposed: tensor = A^T = conj(A)
-        # (A Hermitian), whose eigenvectors are the conjugates of A's
-        w, vt = torch.linalg.eigh(self.A[:n, :n])
-        # (k, n): column-major n x k; conj_physical, because a lazy conj()
-        # view would hand the kernels the unconjugated memory
-        U = torch.conj_physical(vt[:, :k].T.contiguous())
+        if n <= HOST_DENSE_MAX:
+            a = self.host_block(self.A, n)
+            w, v = np.linalg.eigh(0.5 * (a + a.conj().T))
+            U = torch.from_numpy(np.ascontiguousarray(v[:, :k].T)).to(self.device)
+            w = torch.from_numpy(w)
+        else:
+            # the column-major block is stored transposed: tensor = A^T =
+            # conj(A) (A Hermitian), whose eigenvectors are the conjugates of
+            # A's; conj_physical, because a lazy conj() view would hand the
+            # kernels the unconjugated memory.  U: (k, n), column-major n x k
+            w, vt = torch.linalg.eigh(self.A[:n, :n])
+            U = torch.conj_physical(vt[:, :k].T.contiguous())
         coef = torch.empty((k, n), dtype=C128, device=self.device)
         self._gemm(nat.OP_C, n, k, n, 1.0, self.block(self.W, 0, n, 0, n), U, 0.0, coef)
         self._keep_vectors(U, n)
         values = w.cpu().numpy().astype(np.float64)
         self.theta = values
+        self.sigma = None
         return values[:k].copy(), coef
 
     def _keep_vectors(self, U, n):
@@ -263,17 +279,23 @@ class DeviceRayleighRitz:
         st = dv.stream_handle(self.device)
         lib = nat.load()
         theta = self.theta
-        hi = min(len(theta), k + self.extra)
-        # sigma in the widest gap among theta[k-1+margin .. hi-1]: without a
-        # restart since the last solve the subspace only grew, so by Cauchy
-        # interlacing at least k + margin eigenvalues of the new A lie below
-        # sigma and the next solve again knows more than k Ritz values
-        margin = min(4, len(theta) - k - 1)
-        lo_i = k - 1 + margin
-        gaps = np.diff(theta[lo_i:hi])
-        j = int(np.argmax(gaps))
-        lo_v, hi_v = theta[lo_i + j], theta[lo_i + j + 1]
-        sigma = 0.5 * (lo_v + hi_v)
+        if (self.sigma is not None and k + 4 <= len(theta) <= k + self.extra
+                and theta[-1] < self.sigma):
+            # keep the previous shift: the subspace only grew since, so by
+            # Cauchy interlacing at least the len(theta) >= k + 4 eigenvalues
+            # found below it last time still lie below it
+            sigma = self.sigma
+            lo_v, hi_v = theta[-1], 2.0 * sigma - theta[-1]
+        else:
+            # a new shift in the widest gap among theta[k-1+margin .. hi-1]:
+            # at least k + margin eigenvalues of the grown A lie below it
+            hi = min(len(theta), k + self.extra)
+            margin = min(4, len(theta) - k - 1)
+            lo_i = k - 1 + margin
+            gaps = np.diff(theta[lo_i:hi])
+            j = int(np.argmax(gaps))
+            lo_v, hi_v = theta[lo_i + j], theta[lo_i + j + 1]
+            sigma = 0.5 * (lo_v + hi_v)
         A = self.block(self.A, 0, n, 0, n)
         X, X2, Y = (self.block(m, 0, n, 0, n) for m in (self.X, self.X2, self.Y))
         nat.check(lib.qepw_shift_scale(A.data_ptr(), self.maxsub, n, sigma, X.data_ptr(),
@@ -283,19 +305,20 @@ class DeviceRayleighRitz:
         lower = 0.25 * (hi_v - lo_v) / bound
         alphas = _scaled_ns_alphas(lower) + [1.0, 1.0]
         converged = False
+        buf = self.ws.get(int(lib.qepw_sign_iteration_workspace(n)))
+        in_x2 = ctypes.c_int(0)
         for attempt in range(3):
-            for it, a in enumerate(alphas):
-                self._gemm(nat.OP_N, n, n, n, 1.0, X, X, 0.0, Y)
-                X2.copy_(X)
-                self._gemm(nat.OP_N, n, n, n, -0.5 * a ** 3, X, Y, 1.5 * a, X2)
-                # X^3 is Hermitian up to rounding; re-symmetrising every
-                # fourth step (and before the check) keeps the drift at the
-                # rounding level
-                if it % 4 == 3 or it == len(alphas) - 1:
-                    nat.check(lib.qepw_hermitize(X2.data_ptr(), self.maxsub, n, st),
-                              "qepw_hermitize")
+            # the steps run back to back inside the library (X^3 is Hermitian
+            # up to rounding; it re-symmetrises every fourth step and after
+            # the last)
+            coeffs = (ctypes.c_double * len(alphas))(*alphas)
+            nat.check(lib.qepw_sign_iteration(X.data_ptr(), X2.data_ptr(), Y.data_ptr(),
+                                              self.maxsub, n, coeffs, len(alphas),
+                                              ctypes.byref(in_x2), buf.data_ptr(), buf.numel(),
+                                              st), "qepw_sign_iteration")
+            if in_x2.value:
                 X, X2 = X2, X
-                self.stats["ns_iterations"] += 1
+            self.stats["ns_iterations"] += len(alphas)
             self._gemm(nat.OP_N, n, n, n, 1.0, X, X, 0.0, Y)
             nat.check(lib.qepw_sign_check(X.data_ptr(), self.maxsub, Y.data_ptr(), self.maxsub, n,
                                           self.sig.data_ptr(), st), "qepw_sign_check")
@@ -324,36 +347,37 @@ class DeviceRayleighRitz:
         Om = omega[:, :n]
         Q0 = Om.clone()                       # Q0 = (I - X) Omega / 2
         self._gemm(nat.OP_N, n, mp, n, -0.5, X, Om, 0.5, Q0)
-        AQ = torch.empty_like(Q0)
-        self._gemm(nat.OP_N, n, mp, n, 1.0, A, Q0, 0.0, AQ)
-        GB = torch.empty((2, mp, mp), dtype=C128, device=self.device)
-        self._gemm(nat.OP_C, mp, mp, n, 1.0, Q0, Q0, 0.0, GB[0])
-        self._gemm(nat.OP_C, mp, mp, n, 1.0, Q0, AQ, 0.0, GB[1])
-        gb = GB.cpu().numpy()
-        g = gb[0].T
-        b = gb[1].T
+        G = torch.empty((mp, mp), dtype=C128, device=self.device)
+        self._gemm(nat.OP_C, mp, mp, n, 1.0, Q0, Q0, 0.0, G)
+        g = G.cpu().numpy().T
         g = 0.5 * (g + g.conj().T)
-        b = 0.5 * (b + b.conj().T)
         # Q0 = P Omega has exactly rank m_below = mp (Omega: the previous
         # eigenvectors, then Gaussian columns); its Gram matrix is factored by
-        # Cholesky, t = L^-H, so that (Q0 t)^H (Q0 t) = I
+        # Cholesky and Q = Q0 L^-H is an orthonormal basis of range(P)
         try:
             lg = np.linalg.cholesky(g)
-            t = sla.solve_triangular(lg, np.eye(mp), lower=True).conj().T
         except np.linalg.LinAlgError:
             self.stats["gram_not_pd"] += 1
             return None
-        bt = t.conj().T @ b @ t
+        t = sla.solve_triangular(lg, np.eye(mp), lower=True).conj().T
+        t_dev = torch.from_numpy(np.ascontiguousarray(t.T)).to(self.device)
+        Q = torch.empty_like(Q0)
+        self._gemm(nat.OP_N, n, mp, mp, 1.0, Q0, t_dev, 0.0, Q)
+        AQ = torch.empty_like(Q)
+        self._gemm(nat.OP_N, n, mp, n, 1.0, A, Q, 0.0, AQ)
+        B = torch.empty((mp, mp), dtype=C128, device=self.device)
+        self._gemm(nat.OP_C, mp, mp, n, 1.0, Q, AQ, 0.0, B)
+        bt = B.cpu().numpy().T
         bt = 0.5 * (bt + bt.conj().T)
         w, y = np.linalg.eigh(bt)
         if len(w) < k:
             return None
-        zc = t @ y[:, :k]                     # (mp, k)
-        z_dev = torch.from_numpy(np.ascontiguousarray(zc.T)).to(self.device)
+        y_dev = torch.from_numpy(np.ascontiguousarray(y[:, :k].T)).to(self.device)
         U = torch.empty((k, n), dtype=C128, device=self.device)
-        self._gemm(nat.OP_N, n, k, mp, 1.0, Q0, z_dev, 0.0, U)
+        self._gemm(nat.OP_N, n, k, mp, 1.0, Q, y_dev, 0.0, U)
         coef = torch.empty((k, n), dtype=C128, device=self.device)
         self._gemm(nat.OP_C, n, k, n, 1.0, self.block(self.W, 0, n, 0, n), U, 0.0, coef)
         self._keep_vectors(U, n)
         self.theta = np.asarray(w, dtype=np.float64)
+        self.sigma = sigma
         return w[:k].copy(), coef

@@ -73,3 +73,42 @@ def test_spectral_rayleigh_ritz_matches_host(nroots, blocks):
         np.testing.assert_allclose(res, res_ref, atol=1e-9, rtol=1e-6)
         used_spectral = rr.stats["spectral"]
     assert used_spectral > 0, rr.stats
+
+
+def test_sign_iteration_entry_point():
+    """qepw_sign_iteration against the same scaled Newton-Schulz steps in numpy."""
+    import ctypes
+    from qe_python_amd import _native as nat
+    from qe_python_amd import device as dv
+    rng = np.random.default_rng(5)
+    n, ld = 300, 320  # large enough that the square GEMMs split K (workspace > 0)
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    a = a + a.conj().T
+    x0 = a / np.abs(a).sum(axis=0).max()
+    alphas = [1.6, 1.4, 1.2, 1.0, 1.0]
+    ref = x0.copy()
+    for al in alphas:
+        ref = 0.5 * al * ref @ (3.0 * np.eye(n) - al * al * ref @ ref)
+        ref = 0.5 * (ref + ref.conj().T)
+    bufs = [torch.zeros((ld, ld), dtype=torch.complex128, device=DEV) for _ in range(3)]
+    bufs[0][:n, :n] = torch.from_numpy(np.ascontiguousarray(x0.T)).to(DEV)
+    lib = nat.load()
+    ws = dv.Workspace(torch.device(DEV)).get(int(lib.qepw_sign_iteration_workspace(n)))
+    which = ctypes.c_int(-1)
+    coeffs = (ctypes.c_double * len(alphas))(*alphas)
+    st = dv.stream_handle(torch.device(DEV))
+    nat.check(lib.qepw_sign_iteration(bufs[0].data_ptr(), bufs[1].data_ptr(), bufs[2].data_ptr(),
+                                      ld, n, coeffs, len(alphas), ctypes.byref(which),
+                                      ws.data_ptr(), ws.numel(), st), "sign")
+    assert which.value == 1  # five steps: the result ping-ponged into x2
+    got = bufs[which.value][:n, :n].cpu().numpy().T
+    np.testing.assert_allclose(got, ref, atol=1e-12)
+    with pytest.raises(ValueError):
+        nat.check(lib.qepw_sign_iteration(bufs[0].data_ptr(), bufs[0].data_ptr(),
+                                          bufs[2].data_ptr(), ld, n, coeffs, 1,
+                                          ctypes.byref(which), ws.data_ptr(), ws.numel(), st),
+                  "sign")
+    with pytest.raises(RuntimeError):
+        nat.check(lib.qepw_sign_iteration(bufs[0].data_ptr(), bufs[1].data_ptr(),
+                                          bufs[2].data_ptr(), ld, n, coeffs, 1,
+                                          ctypes.byref(which), ws.data_ptr(), 0, st), "sign")
