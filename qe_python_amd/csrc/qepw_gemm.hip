@@ -77,32 +77,56 @@ __global__ __launch_bounds__(kGemmThreads) void k_cgemm(
 #pragma unroll
       for (int v = 0; v < 4; ++v) { re[i][j][v] = T(0); im[i][j][v] = T(0); }
 
+  // Software pipeline: the operands of step k0 + kBK are fetched into
+  // registers while the MFMAs of step k0 run from LDS, so the global-load
+  // latency overlaps the matrix work instead of preceding it.
+  constexpr int kPer = kBK * TILE / kGemmThreads;  // elements per thread and operand
+  cplx<T> ra[kPer], rb[kPer];
+  // element e of the kBK x TILE operand tiles of step K0
+  auto load_a = [=](int e, int64_t k0) -> cplx<T> {
+    const int kk = CONJ_A ? (e & (kBK - 1)) : (e / TILE);
+    const int mm = CONJ_A ? (e / kBK) : (e & (TILE - 1));
+    const int64_t ka = k0 + kk, m = m0 + mm;
+    if (ka >= kend || m >= M) return zero;
+    return CONJ_A ? A[ka + m * lda] : A[m + ka * lda];
+  };
+  auto load_b = [=](int e, int64_t k0) -> cplx<T> {
+    const int kb = CONJ_B ? (e / TILE) : (e & (kBK - 1));
+    const int nb = CONJ_B ? (e & (TILE - 1)) : (e / kBK);
+    const int64_t kq = k0 + kb, n = n0 + nb;
+    if (kq >= kend || n >= N) return zero;
+    if (CONJ_B) {  // op(B) = B^H, B stored n x k
+      cplx<T> w = B[n + kq * ldb];
+      w.y = -w.y;
+      return w;
+    }
+    return B[kq + n * ldb];
+  };
+
+  if (kbeg < kend) {
+#pragma unroll
+    for (int p = 0; p < kPer; ++p) {
+      ra[p] = load_a(tid + p * kGemmThreads, kbeg);
+      rb[p] = load_b(tid + p * kGemmThreads, kbeg);
+    }
+  }
   for (int64_t k0 = kbeg; k0 < kend; k0 += kBK) {
 #pragma unroll
-    for (int e = tid; e < kBK * TILE; e += kGemmThreads) {
-      int kk, mm;
-      if (CONJ_A) { kk = e & (kBK - 1); mm = e / kBK; }
-      else { mm = e & (TILE - 1); kk = e / TILE; }
-      const int64_t k = k0 + kk, m = m0 + mm;
-      cplx<T> v = zero;
-      if (k < kend && m < M) v = CONJ_A ? A[k + m * lda] : A[m + k * lda];
-      As[kk][mm] = v;
-      int kb, nb;
-      if (CONJ_B) { nb = e & (TILE - 1); kb = e / TILE; }
-      else { kb = e & (kBK - 1); nb = e / kBK; }
-      const int64_t kq = k0 + kb, n = n0 + nb;
-      cplx<T> w = zero;
-      if (kq < kend && n < N) {
-        if (CONJ_B) {  // op(B) = B^H, B stored n x k
-          w = B[n + kq * ldb];
-          w.y = -w.y;
-        } else {
-          w = B[kq + n * ldb];
-        }
-      }
-      Bs[kb][nb] = w;
+    for (int p = 0; p < kPer; ++p) {
+      const int e = tid + p * kGemmThreads;
+      if (CONJ_A) As[e & (kBK - 1)][e / kBK] = ra[p];
+      else As[e / TILE][e & (TILE - 1)] = ra[p];
+      if (CONJ_B) Bs[e / TILE][e & (TILE - 1)] = rb[p];
+      else Bs[e & (kBK - 1)][e / kBK] = rb[p];
     }
     __syncthreads();
+    if (k0 + kBK < kend) {
+#pragma unroll
+      for (int p = 0; p < kPer; ++p) {
+        ra[p] = load_a(tid + p * kGemmThreads, k0 + kBK);
+        rb[p] = load_b(tid + p * kGemmThreads, k0 + kBK);
+      }
+    }
 #pragma unroll
     for (int ks = 0; ks < kBK; ks += 4) {
       const int kk = ks + (lane >> 4);
