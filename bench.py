@@ -318,10 +318,18 @@ def main():
     total_ms = sum(a.elapsed_time(b) for _, _, a, b in head_events)
     calls = len(head_events)
     achieved = total_bytes / (total_ms * 1e-3) if total_ms > 0 else 0.0
+    # HBM bytes per launch: the traffic/algorithmic ratio of the committed
+    # rocprofv3 FETCH_SIZE/WRITE_SIZE passes (128-band FP64 calls) applied to
+    # this run's average algorithmic bytes per call
     traffic = None
     pmc = ROOT / "profiles" / "fft_phase_medium_traffic.json"
     if pmc.exists():
-        traffic = json.loads(pmc.read_text()).get("traffic_bytes_per_call")
+        probe = json.loads(pmc.read_text())
+        ratio = probe.get("traffic_over_algorithmic")
+        if ratio is None and probe.get("algorithmic_bytes_per_call"):
+            ratio = probe["traffic_bytes_per_call"] / probe["algorithmic_bytes_per_call"]
+        if ratio is not None and calls:
+            traffic = ratio * total_bytes / calls
 
     line = {
         "metric": "Davidson solves/s to 1e-8 Ry (mixed vs FP64 GPU vs CPU); h_psi HBM GB/s",
