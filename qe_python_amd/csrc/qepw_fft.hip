@@ -37,20 +37,24 @@ __global__ __launch_bounds__(kStickThreads) void k_sticks_inverse(
   const int ns = min(group, nst - s0);
   const int64_t b = blockIdx.y;
   const int len = ns * n3;
+  const int lsd = n3 + 1;  // padded line stride: lines start on different LDS banks
   const cplx<T> zero = mk<T>(T(0), T(0));
-  for (int i = tid; i < len; i += kStickThreads) L[i] = zero;
+  for (int i = tid; i < ns * lsd; i += kStickThreads) L[i] = zero;
   __syncthreads();
   const int d0 = off[s0], d1 = off[s0 + ns];
   const int shift = s0 * n3;
   const cplx<T>* src = psi + b * ldpsi;
-  for (int d = d0 + tid; d < d1; d += kStickThreads) L[gpos[d] - shift] = src[d];
+  for (int d = d0 + tid; d < d1; d += kStickThreads) {
+    const int q = gpos[d] - shift;  // l * n3 + z
+    L[q + q / n3] = src[d];
+  }
   __syncthreads();
-  lds_fft<T, kStickThreads, kStickPts>(L, ns, n3, 1, nullptr, pz, tw, +1, tid);
+  lds_fft<T, kStickThreads, kStickPts>(L, ns, lsd, 1, nullptr, pz, tw, +1, tid);
   cplx<T>* w = work + b * band_stride + s0;
   for (int i = tid; i < len; i += kStickThreads) {
     const int z = i / ns;
     const int l = i - z * ns;
-    w[(int64_t)z * nst + l] = L[l * n3 + z];
+    w[(int64_t)z * nst + l] = L[l * lsd + z];
   }
 }
 
@@ -68,20 +72,22 @@ __global__ __launch_bounds__(kStickThreads) void k_sticks_forward(
   const int ns = min(group, nst - s0);
   const int64_t b = blockIdx.y;
   const int len = ns * n3;
+  const int lsd = n3 + 1;  // padded line stride (see k_sticks_inverse)
   const cplx<T>* w = work + b * band_stride + s0;
   for (int i = tid; i < len; i += kStickThreads) {
     const int z = i / ns;
     const int l = i - z * ns;
-    L[l * n3 + z] = w[(int64_t)z * nst + l];
+    L[l * lsd + z] = w[(int64_t)z * nst + l];
   }
   __syncthreads();
-  lds_fft<T, kStickThreads, kStickPts>(L, ns, n3, 1, nullptr, pz, tw, -1, tid);
+  lds_fft<T, kStickThreads, kStickPts>(L, ns, lsd, 1, nullptr, pz, tw, -1, tid);
   const int d0 = off[s0], d1 = off[s0 + ns];
   const int shift = s0 * n3;
   const cplx<T>* src = psi + b * ldpsi;
   cplx<T>* dst = out + b * ldout;
   for (int d = d0 + tid; d < d1; d += kStickThreads) {
-    cplx<T> v = cscale(L[gpos[d] - shift], inv_n);
+    const int q = gpos[d] - shift;
+    cplx<T> v = cscale(L[q + q / n3], inv_n);
     if (kinetic) {
       const T k = kinetic[d];
       const cplx<T> p = src[d];
@@ -159,18 +165,21 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
     const T* __restrict__ V, const cplx<T>* __restrict__ tw) {
   using G = TeamGeom<N>;
   extern __shared__ unsigned char smem_raw[];
+  // Q[c * NP + y]: active column c, padded stride NP = N + 1 so that the
+  // columns (and the teams' scratch lines) start on different LDS banks
+  constexpr int NP = N + 1;
   cplx<T>* Q = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* scratch_all = Q + nxc * N;
+  cplx<T>* scratch_all = Q + nxc * NP;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int team_in_wave = lane / G::M;
   const int t = lane - team_in_wave * G::M;
   const bool in_team = team_in_wave < G::per_wave;
   const int team = wave * G::per_wave + team_in_wave;
-  cplx<T>* S = scratch_all + team * N;
+  cplx<T>* S = scratch_all + team * NP;
   const int z = blockIdx.x;
   const int64_t b = blockIdx.y;
-  const int nq = nxc * N;
+  const int nq = nxc * NP;
   const cplx<T> zero = mk<T>(T(0), T(0));
   cplx<T>* w = work + b * band_stride + (int64_t)z * nst;
 
@@ -187,13 +196,16 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
 
   for (int i = tid; i < nq; i += kTeamThreads) Q[i] = zero;
   __syncthreads();
-  for (int s = tid; s < nst; s += kTeamThreads) Q[stick_q[s]] = w[s];
+  for (int s = tid; s < nst; s += kTeamThreads) {
+    const int q = stick_q[s];  // c * N + y
+    Q[q + q / N] = w[s];
+  }
   __syncthreads();
 
   // backward y-FFT of every active column (in place in Q)
   if (in_team) {
     for (int c = team; c < nxc; c += G::teams) {
-      cplx<T>* col = Q + c * N;
+      cplx<T>* col = Q + c * NP;
       team_fft<T, N, +1>(t, S, twr, [&](int j, int) { return col[j]; },
                          [&](int k, int, cplx<T> v) { col[k] = v; });
     }
@@ -211,7 +223,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
       for (int s = 0; s < G::L; ++s) vv[s] = t < G::M ? vz[y * N + t + G::M * s] : T(0);
       team_fft<T, N, +1>(
           t, S, twr,
-          [&](int, int s) { return cin[s] >= 0 ? Q[cin[s] * N + y] : zero; },
+          [&](int, int s) { return cin[s] >= 0 ? Q[cin[s] * NP + y] : zero; },
           [&](int k, int s, cplx<T> v) { S[k] = cscale(v, vv[s]); });
       // S now holds V psi of this row in natural order (written after the
       // team's last exchange read, read back before any exchange write)
@@ -219,7 +231,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
       team_fft<T, N, -1>(
           t, S, twr, [&](int j, int) { return S[j]; },
           [&](int, int s, cplx<T> v) {
-            if (cout[s] >= 0) Q[cout[s] * N + y] = v;
+            if (cout[s] >= 0) Q[cout[s] * NP + y] = v;
           });
     }
   }
@@ -228,13 +240,16 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
   // forward y-FFT of every active column
   if (in_team) {
     for (int c = team; c < nxc; c += G::teams) {
-      cplx<T>* col = Q + c * N;
+      cplx<T>* col = Q + c * NP;
       team_fft<T, N, -1>(t, S, twr, [&](int j, int) { return col[j]; },
                          [&](int k, int, cplx<T> v) { col[k] = v; });
     }
   }
   __syncthreads();
-  for (int s = tid; s < nst; s += kTeamThreads) w[s] = Q[stick_q[s]];
+  for (int s = tid; s < nst; s += kTeamThreads) {
+    const int q = stick_q[s];
+    w[s] = Q[q + q / N];
+  }
 }
 
 // Square plane sizes with an instantiated fast kernel.
@@ -248,7 +263,7 @@ static size_t team_lds_bytes(int n, int nxc) {
 #define QEPW_TEAMS_OF(NN) if (n == NN) teams = TeamGeom<NN>::teams;
   QEPW_TEAM_SIZES(QEPW_TEAMS_OF)
 #undef QEPW_TEAMS_OF
-  return ((size_t)nxc * n + teams * n) * sizeof(cplx<T>);
+  return ((size_t)nxc + teams) * (size_t)(n + 1) * sizeof(cplx<T>);
 }
 
 // Launches the fast plane kernel if one exists for this plane; returns false
@@ -339,7 +354,7 @@ int apply_local_t(const Plan& p, const void* vectors, int64_t ldv, int nb,
   int chunk = (int)(ws_bytes / band_bytes);
   if (chunk < 1) return QEPW_ERR_WORKSPACE;
   if (chunk > 65535) chunk = 65535;
-  const size_t stick_lds = (size_t)st.group * p.n3 * sizeof(cplx<T>);
+  const size_t stick_lds = (size_t)st.group * (p.n3 + 1) * sizeof(cplx<T>);
   const size_t plane_lds = (size_t)p.n1 * p.n2 * sizeof(cplx<T>);
   set_lds_attrs<T>(stick_lds, team ? 0 : plane_lds);
   const int64_t bstride = (int64_t)p.n3 * st.nst;
@@ -395,21 +410,22 @@ __global__ __launch_bounds__(kTeamThreads) void k_density_team(
     int accumulate, const cplx<T>* __restrict__ tw) {
   using G = TeamGeom<N>;
   extern __shared__ unsigned char smem_raw[];
+  constexpr int NP = N + 1;  // padded column stride, as in k_planes_team
   cplx<T>* Q = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* scratch_all = Q + nxc * N;
-  double* D = reinterpret_cast<double*>(scratch_all + G::teams * N);
+  cplx<T>* scratch_all = Q + nxc * NP;
+  double* D = reinterpret_cast<double*>(scratch_all + G::teams * NP);
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int team_in_wave = lane / G::M;
   const int t = lane - team_in_wave * G::M;
   const bool in_team = team_in_wave < G::per_wave;
   const int team = wave * G::per_wave + team_in_wave;
-  cplx<T>* S = scratch_all + team * N;
+  cplx<T>* S = scratch_all + team * NP;
   const int z = blockIdx.x;
   const int slice = blockIdx.y;
   const int b0 = slice * bands_per_slice;
   const int b1 = min(nb, b0 + bands_per_slice);
-  const int nq = nxc * N;
+  const int nq = nxc * NP;
   const cplx<T> zero = mk<T>(T(0), T(0));
   TeamTwiddles<T, N> twr;
   twr.load(t, tw);
@@ -422,11 +438,14 @@ __global__ __launch_bounds__(kTeamThreads) void k_density_team(
     const double wb = weights[b];
     for (int i = tid; i < nq; i += kTeamThreads) Q[i] = zero;
     __syncthreads();
-    for (int s = tid; s < nst; s += kTeamThreads) Q[stick_q[s]] = w[s];
+    for (int s = tid; s < nst; s += kTeamThreads) {
+      const int q = stick_q[s];
+      Q[q + q / N] = w[s];
+    }
     __syncthreads();
     if (in_team) {
       for (int c = team; c < nxc; c += G::teams) {
-        cplx<T>* col = Q + c * N;
+        cplx<T>* col = Q + c * NP;
         team_fft<T, N, +1>(t, S, twr, [&](int j, int) { return col[j]; },
                            [&](int k, int, cplx<T> v) { col[k] = v; });
       }
@@ -437,7 +456,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_density_team(
         double* drow = D + y * N;
         team_fft<T, N, +1>(
             t, S, twr,
-            [&](int, int s) { return cin[s] >= 0 ? Q[cin[s] * N + y] : zero; },
+            [&](int, int s) { return cin[s] >= 0 ? Q[cin[s] * NP + y] : zero; },
             [&](int k, int, cplx<T> v) {
               drow[k] += wb * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
             });
@@ -545,7 +564,7 @@ int accumulate_density_t(const Plan& p, const void* vectors, int64_t ldv, int nb
   cplx<T>* work = reinterpret_cast<cplx<T>*>(static_cast<char*>(workspace) + partial_bytes);
   int chunk = (int)((ws_bytes - partial_bytes) / band_bytes);
   if (chunk > 65535) chunk = 65535;
-  const size_t stick_lds = (size_t)st.group * p.n3 * sizeof(cplx<T>);
+  const size_t stick_lds = (size_t)st.group * (p.n3 + 1) * sizeof(cplx<T>);
   const size_t plane_lds = np * sizeof(cplx<T>);
   set_lds_attrs<T>(stick_lds, 0);
   if (!team)
@@ -616,7 +635,7 @@ int inverse_real(const Plan& p, const void* grid_g, void* out_real, void* worksp
   int rc = check_fft_limits<double>(p, st);
   if (rc) return rc;
   if (ws_bytes < inverse_real_bytes(p)) return QEPW_ERR_WORKSPACE;
-  const size_t stick_lds = (size_t)st.group * p.n3 * sizeof(double2);
+  const size_t stick_lds = (size_t)st.group * (p.n3 + 1) * sizeof(double2);
   const size_t plane_lds = (size_t)p.n1 * p.n2 * sizeof(double2);
   set_lds_attrs<double>(stick_lds, plane_lds);
   const int64_t bstride = (int64_t)p.n3 * st.nst;
