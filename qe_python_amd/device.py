@@ -13,6 +13,8 @@ F-ordered ``basis_storage[:, a:b]``, ``qepy_pw/diagonalization.py:632-637``).
 from __future__ import annotations
 
 import ctypes
+import threading
+from contextlib import contextmanager
 
 import numpy as np
 import torch
@@ -35,8 +37,40 @@ def real_dtype(dtype: torch.dtype) -> torch.dtype:
     return torch.float64 if dtype == C128 else torch.float32
 
 
+_TLS = threading.local()
+
+
 def stream_handle(device: torch.device) -> int:
+    """HIP stream handle of the current torch stream (the one fixed by an
+    enclosing ``fixed_stream`` scope of this thread, if any)."""
+    fixed = getattr(_TLS, "stream", None)
+    if fixed is not None:
+        return fixed
     return torch.cuda.current_stream(device).cuda_stream
+
+
+@contextmanager
+def fixed_stream(device: torch.device):
+    """Within one solve the current stream does not change: look it up once
+    instead of per launch (thread-local, so concurrent solves on their own
+    streams are unaffected)."""
+    prev = getattr(_TLS, "stream", None)
+    _TLS.stream = torch.cuda.current_stream(device).cuda_stream
+    try:
+        yield
+    finally:
+        _TLS.stream = prev
+
+
+def upload(array, device: torch.device) -> torch.Tensor:
+    """Small host array -> device tensor without stalling the host: staged in
+    pinned memory (torch's caching host allocator keeps the staging block
+    until the copy has run) and copied asynchronously on the current stream;
+    a pageable copy would wait for all queued device work first."""
+    src = torch.from_numpy(np.ascontiguousarray(array))
+    pinned = torch.empty(src.shape, dtype=src.dtype, pin_memory=True)
+    pinned.copy_(src)
+    return pinned.to(device, non_blocking=True)
 
 
 def require_device(device=None) -> torch.device:

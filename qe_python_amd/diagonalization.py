@@ -495,7 +495,7 @@ def _cholesky_orthonormalize(block: torch.Tensor, npw: int, ws, condition: float
         if d.min() <= 0.0 or (d.max() / d.min()) ** 2 > condition:
             return None
         inv = sla.solve_triangular(chol, np.eye(n), lower=True).conj().T   # L^-H
-    t_dev = torch.from_numpy(np.ascontiguousarray(inv.T)).to(device=device, dtype=block.dtype)
+    t_dev = dv.upload(inv.T, device).to(block.dtype)
     out = torch.empty_like(block)
     dv.gemm(nat.OP_N, npw, n, n, 1.0, block, t_dev, 0.0, out, ws)
     return out
@@ -529,7 +529,7 @@ def _orthonormalize_device(block: torch.Tensor, npw: int, ws, threshold: float =
         return block[:0].clone()
     rank = int(np.count_nonzero(squared > threshold ** 2 * squared[0]))
     transform = rotation[:, order[:rank]] / np.sqrt(squared[:rank])[None, :]
-    t_dev = torch.from_numpy(np.ascontiguousarray(transform.T)).to(device=device, dtype=block.dtype)
+    t_dev = dv.upload(transform.T, device).to(block.dtype)
     out = torch.empty((rank, npw), dtype=block.dtype, device=device)
     dv.gemm(nat.OP_N, npw, rank, n, 1.0, block, t_dev, 0.0, out, ws)
     return out
@@ -703,10 +703,15 @@ class _PhaseResult:
     happ: int
 
 
-def _davidson_phase(ctx, start, dtype, energy_bounds, residual_bounds, residual_energy_scale,
-                    root_tolerances, max_iterations, start_is_ritz, start_is_orthonormal,
-                    initial_eigenvalues, applied_block, perm_dev, *, stall_window=None,
-                    require_residuals=False, coarse=False):
+def _davidson_phase(ctx, *args, **kwargs):
+    with dv.fixed_stream(ctx.device):
+        return _davidson_phase_body(ctx, *args, **kwargs)
+
+
+def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, residual_energy_scale,
+                         root_tolerances, max_iterations, start_is_ritz, start_is_orthonormal,
+                         initial_eigenvalues, applied_block, perm_dev, *, stall_window=None,
+                         require_residuals=False, coarse=False):
     """The restarted Davidson loop (``diagonalization.py:640-880``) at one
     storage dtype, from a device-order start block ``start`` (nb, npw).
 
@@ -768,7 +773,7 @@ def _davidson_phase(ctx, start, dtype, energy_bounds, residual_bounds, residual_
             R = torch.empty_like(X)
             dv.gemm(nat.OP_N, npw, number_of_roots, active, 1.0, V[:active], coef, 0.0, X, ws)
             dv.gemm(nat.OP_N, npw, number_of_roots, active, 1.0, HV[:active], coef, 0.0, R, ws)
-            e = torch.from_numpy(np.ascontiguousarray(roots, dtype=np.float64)).to(device)
+            e = dv.upload(np.asarray(roots, dtype=np.float64), device)
             dv.subtract_band_energies(R, X, e, npw)
         return X, R
 
@@ -776,9 +781,9 @@ def _davidson_phase(ctx, start, dtype, energy_bounds, residual_bounds, residual_
         """Residuals of the roots ``cols`` only, packed:
         R_u = HV C_u - V (C_u diag(e_u)) -- all the next expansion needs."""
         with _measure(timers, "cegterg:upda", device):
-            idx = torch.from_numpy(cols.astype(np.int64)).to(device)
+            idx = dv.upload(cols.astype(np.int64), device)
             cu = coef.index_select(0, idx)
-            e = torch.from_numpy(np.ascontiguousarray(roots[cols], dtype=np.float64)).to(device)
+            e = dv.upload(np.asarray(roots[cols], dtype=np.float64), device)
             ce = cu * e.to(cu.real.dtype)[:, None]
             R = torch.empty((len(cols), npw), dtype=dtype, device=device)
             dv.gemm(nat.OP_N, npw, len(cols), active, 1.0, HV[:active], cu, 0.0, R, ws)
@@ -823,8 +828,8 @@ def _davidson_phase(ctx, start, dtype, energy_bounds, residual_bounds, residual_
         with _measure(timers, "g_psi", device):
             cols = np.flatnonzero(unconverged).astype(np.int32)
             rows = np.arange(len(cols), dtype=np.int32) if packed else cols
-            rows_dev = torch.from_numpy(rows).to(device)
-            e_dev = torch.from_numpy(np.ascontiguousarray(values[cols])).to(device)
+            rows_dev = dv.upload(rows, device)
+            e_dev = dv.upload(values[cols], device)
             corrections = torch.empty((len(cols), npw), dtype=dtype, device=device)
             dv.qe_precondition(R, rows_dev, e_dev, diag_dev, corrections, npw)
         if coarse and len(cols):
@@ -862,7 +867,7 @@ def _davidson_phase(ctx, start, dtype, energy_bounds, residual_bounds, residual_
                 break
             old = active
             active += len(keep)
-            dv.normalize_selected_columns(corrections, sq_dev, torch.from_numpy(keep).to(device),
+            dv.normalize_selected_columns(corrections, sq_dev, dv.upload(keep, device),
                                           V[old:active], npw)
         del X, R, corrections
         X = None
@@ -913,7 +918,7 @@ def _davidson_phase(ctx, start, dtype, energy_bounds, residual_bounds, residual_
                     packed = False
                 active = number_of_roots
                 V[:active].copy_(X)
-                e = torch.from_numpy(np.ascontiguousarray(values)).to(device)
+                e = dv.upload(values, device)
                 dv.restart_applied(X, R, e, HV[:active], npw)
                 rr.restart(values)
         elif X is None:

@@ -97,6 +97,7 @@ class DeviceRayleighRitz:
         self.standard = 0            # W and A are valid for [:standard]
         self.theta = None            # previous Ritz values (host, ascending)
         self.sigma = None            # shift of the previous spectral solve
+        self._bound_prev = None      # ||A - sigma I||_1 of the previous solve (host)
         self.u_prev = None           # previous lowest eigenvectors of A (device, (k, maxsub))
         self.stats = {"spectral": 0, "dense": 0, "host": 0, "ns_iterations": 0,
                       "max_check_error": 0.0, "skipped": 0, "ns_unconverged": 0,
@@ -137,7 +138,7 @@ class DeviceRayleighRitz:
     def restart(self, values: np.ndarray):
         """Restart block (diagonalization.py:843-868): H = diag(e), S = I."""
         k = len(values)
-        e = torch.from_numpy(np.ascontiguousarray(values, dtype=np.float64)).to(self.device)
+        e = dv.upload(np.asarray(values, dtype=np.float64), self.device)
         lib = nat.load()
         st = dv.stream_handle(self.device)
         for mat, d in ((self.H, e.data_ptr()), (self.S, None), (self.W, None),
@@ -175,7 +176,7 @@ class DeviceRayleighRitz:
         except np.linalg.LinAlgError as exc:
             raise np.linalg.LinAlgError(f"zhegvd failed: overlap not positive definite ({exc})")
         w22 = sla.solve_triangular(l22, np.eye(nnew), lower=True)
-        w22_dev = torch.from_numpy(np.ascontiguousarray(w22.T)).to(self.device)
+        w22_dev = dv.upload(w22.T, self.device)
         self.block(self.W, first, active, first, active).copy_(w22_dev)
         # the strictly upper part of the new columns stays zero
         self.block(self.W, 0, first, first, active).zero_()
@@ -250,7 +251,7 @@ class DeviceRayleighRitz:
         if n <= HOST_DENSE_MAX:
             a = self.host_block(self.A, n)
             w, v = np.linalg.eigh(0.5 * (a + a.conj().T))
-            U = torch.from_numpy(np.ascontiguousarray(v[:, :k].T)).to(self.device)
+            U = dv.upload(v[:, :k].T, self.device)
             w = torch.from_numpy(w)
         else:
             # the column-major block is stored transposed: tensor = A^T =
@@ -301,55 +302,86 @@ class DeviceRayleighRitz:
         nat.check(lib.qepw_shift_scale(A.data_ptr(), self.maxsub, n, sigma, X.data_ptr(),
                                        self.maxsub, self.bound.data_ptr(), st),
                   "qepw_shift_scale")
-        bound = float(self.bound[:1].cpu()[0])
-        lower = 0.25 * (hi_v - lo_v) / bound
+        # the scaling schedule needs ||A - sigma I||_1 only as an estimate
+        # (the device scales X0 by the exact value; the scaled steps map
+        # (0, 1] into itself for any schedule, a poor one just converges
+        # later): twice the previous solve's value avoids a read-back here
+        if self._bound_prev is None:
+            self._bound_prev = float(self.bound[:1].cpu()[0])
+            bound_est = self._bound_prev
+        else:
+            bound_est = 2.0 * self._bound_prev
+        lower = 0.25 * (hi_v - lo_v) / bound_est
         alphas = _scaled_ns_alphas(lower) + [1.0, 1.0]
-        converged = False
         buf = self.ws.get(int(lib.qepw_sign_iteration_workspace(n)))
         in_x2 = ctypes.c_int(0)
-        for attempt in range(3):
+
+        def sign_steps(schedule):
+            nonlocal X, X2
             # the steps run back to back inside the library (X^3 is Hermitian
             # up to rounding; it re-symmetrises every fourth step and after
             # the last)
-            coeffs = (ctypes.c_double * len(alphas))(*alphas)
+            coeffs = (ctypes.c_double * len(schedule))(*schedule)
             nat.check(lib.qepw_sign_iteration(X.data_ptr(), X2.data_ptr(), Y.data_ptr(),
-                                              self.maxsub, n, coeffs, len(alphas),
+                                              self.maxsub, n, coeffs, len(schedule),
                                               ctypes.byref(in_x2), buf.data_ptr(), buf.numel(),
                                               st), "qepw_sign_iteration")
             if in_x2.value:
                 X, X2 = X2, X
-            self.stats["ns_iterations"] += len(alphas)
+            self.stats["ns_iterations"] += len(schedule)
             self._gemm(nat.OP_N, n, n, n, 1.0, X, X, 0.0, Y)
             nat.check(lib.qepw_sign_check(X.data_ptr(), self.maxsub, Y.data_ptr(), self.maxsub, n,
                                           self.sig.data_ptr(), st), "qepw_sign_check")
+
+        def sampled_range(count):
+            """Q0 = (I - X) Omega / 2 for Omega = [previous eigenvectors,
+            Gaussian columns] (count columns) and its Gram matrix."""
+            omega = torch.zeros((count, self.maxsub), dtype=C128, device=self.device)
+            start = 0
+            if self.u_prev is not None:
+                start = min(k, count)
+                omega[:start].copy_(self.u_prev[:start])
+            if start < count:
+                r = self._rng.standard_normal((count - start, n)) + 1j * self._rng.standard_normal(
+                    (count - start, n))
+                omega[start:count, :n] = dv.upload(r, self.device)
+            Om = omega[:, :n]
+            Q0 = Om.clone()
+            self._gemm(nat.OP_N, n, count, n, -0.5, X, Om, 0.5, Q0)
+            G = torch.empty((count, count), dtype=C128, device=self.device)
+            self._gemm(nat.OP_C, count, count, n, 1.0, Q0, Q0, 0.0, G)
+            return Q0, G
+
+        # optimistic path: one read-back carries the sign check, the exact
+        # bound and the Gram matrix of a range sample sized for the expected
+        # number of eigenvalues below sigma
+        sign_steps(alphas)
+        count = min(n, len(theta) + 24)
+        Q0, G = sampled_range(count)
+        packed = torch.cat([self.sig[:2], self.bound[:1],
+                            torch.view_as_real(G).reshape(-1)]).cpu().numpy()
+        res2, trace, self._bound_prev = (float(v) for v in packed[:3])
+        g = packed[3:].view(np.complex128).reshape(count, count).T
+        attempt = 0
+        while res2 >= 1e-20 * n and attempt < 2:
+            sign_steps([1.0] * 12)
             res2, trace = (float(v) for v in self.sig[:2].cpu().numpy())
-            if res2 < 1e-20 * n:
-                converged = True
-                break
-            alphas = [1.0] * 12
-        if not converged:
+            attempt += 1
+            g = None
+        if res2 >= 1e-20 * n:
             self.stats["ns_unconverged"] += 1
             return None
         m_below = int(round(0.5 * (n - trace)))
         if m_below < k:
             self.stats["too_few_below"] += 1
             return None
+        if g is None or m_below > count:
+            count = m_below
+            Q0, G = sampled_range(count)
+            g = G.cpu().numpy().T
         mp = m_below
-        omega = torch.zeros((mp, self.maxsub), dtype=C128, device=self.device)
-        start = 0
-        if self.u_prev is not None:
-            start = min(k, mp)
-            omega[:start].copy_(self.u_prev[:start])
-        if start < mp:
-            r = self._rng.standard_normal((mp - start, n)) + 1j * self._rng.standard_normal(
-                (mp - start, n))
-            omega[start:mp, :n] = torch.from_numpy(r).to(self.device)
-        Om = omega[:, :n]
-        Q0 = Om.clone()                       # Q0 = (I - X) Omega / 2
-        self._gemm(nat.OP_N, n, mp, n, -0.5, X, Om, 0.5, Q0)
-        G = torch.empty((mp, mp), dtype=C128, device=self.device)
-        self._gemm(nat.OP_C, mp, mp, n, 1.0, Q0, Q0, 0.0, G)
-        g = G.cpu().numpy().T
+        Q0 = Q0[:mp]
+        g = g[:mp, :mp]
         g = 0.5 * (g + g.conj().T)
         # Q0 = P Omega has exactly rank m_below = mp (Omega: the previous
         # eigenvectors, then Gaussian columns); its Gram matrix is factored by
@@ -360,7 +392,7 @@ class DeviceRayleighRitz:
             self.stats["gram_not_pd"] += 1
             return None
         t = sla.solve_triangular(lg, np.eye(mp), lower=True).conj().T
-        t_dev = torch.from_numpy(np.ascontiguousarray(t.T)).to(self.device)
+        t_dev = dv.upload(t.T, self.device)
         Q = torch.empty_like(Q0)
         self._gemm(nat.OP_N, n, mp, mp, 1.0, Q0, t_dev, 0.0, Q)
         AQ = torch.empty_like(Q)
@@ -372,7 +404,7 @@ class DeviceRayleighRitz:
         w, y = np.linalg.eigh(bt)
         if len(w) < k:
             return None
-        y_dev = torch.from_numpy(np.ascontiguousarray(y[:, :k].T)).to(self.device)
+        y_dev = dv.upload(y[:, :k].T, self.device)
         U = torch.empty((k, n), dtype=C128, device=self.device)
         self._gemm(nat.OP_N, n, k, mp, 1.0, Q, y_dev, 0.0, U)
         coef = torch.empty((k, n), dtype=C128, device=self.device)
