@@ -62,6 +62,11 @@ MIXED_SWITCH = (1.0e-5, 1.0e-5)
 # ordinary slow tail of the top roots and the cheap phase continues)
 MIXED_STALL_WINDOW = 4
 MIXED_STALL_SPAN = 100.0
+# the complex64 phase locks, at a restart, the converged Ritz pairs at the
+# bottom of the spectrum (at least this many, ending at a gap of at least
+# MIXED_LOCK_GAP) and continues on the others only
+MIXED_LOCK_MIN = 8
+MIXED_LOCK_GAP = 1.0e-3     # Ha
 # diagnostics: set to a list to receive one record per Davidson iteration
 # (storage dtype, iteration, unconverged roots, max |de|, max residual norm
 # or None when residuals are not formed)
@@ -718,7 +723,11 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
     ``coarse``: the complex64 phase of the mixed form.  When its FP32
     overlap loses definiteness (new corrections dependent at FP32
     precision) the phase ends on the previous Ritz pairs instead of raising
-    ``LinAlgError``; the FP64 refinement continues from them.
+    ``LinAlgError``; the FP64 refinement continues from them.  At a restart
+    it locks the converged Ritz pairs at the bottom of the spectrum (a run of
+    at least ``MIXED_LOCK_MIN`` ending at a gap of ``MIXED_LOCK_GAP``): they
+    leave the search space, later corrections are orthogonalised against
+    them, and the reduced problem shrinks to the remaining roots.
 
     A root converges when |de| < ``energy_bounds`` and, when given, its
     residual norm < ``residual_bounds`` and its squared residual norm <
@@ -728,6 +737,22 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
     over that many iterations."""
     op, ws, device, npw, timers = ctx.op, ctx.ws, ctx.device, ctx.npw, ctx.timers
     number_of_roots = ctx.nroots
+    # roots still in the search space (all of them unless the complex64
+    # phase has locked some) and their per-root bounds
+    gidx = np.arange(number_of_roots)
+    eb, rb, rt = energy_bounds, residual_bounds, root_tolerances
+    locked = None                 # locked Ritz vectors (device rows)
+    lock_vals, lock_res = [], []
+
+    def merged(values, X, residual_norms):
+        """Locked and active Ritz pairs together, by ascending value."""
+        if locked is None:
+            return values, X, residual_norms
+        allv = np.concatenate(lock_vals + [values])
+        allr = np.concatenate(lock_res + [residual_norms])
+        order = np.argsort(allv, kind="stable")
+        Xall = torch.cat([locked, X]).index_select(0, dv.upload(order.astype(np.int64), device))
+        return allv[order], Xall, allr[order]
     if start_is_ritz or start_is_orthonormal:
         basis = start[:number_of_roots]
     else:
@@ -766,6 +791,17 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
             coef = coef.to(dtype)
         return roots, coef
 
+    def deflate(R):
+        """With locked pairs the search runs on the operator projected onto
+        their complement (the basis is orthogonal to them): its residuals are
+        R - L (L^H R)."""
+        if locked is not None:
+            nl = int(locked.shape[0])
+            proj = torch.empty((R.shape[0], nl), dtype=dtype, device=device)
+            dv.gemm(nat.OP_C, nl, R.shape[0], npw, 1.0, locked, R, 0.0, proj, ws)
+            dv.gemm(nat.OP_N, npw, R.shape[0], nl, -1.0, locked, proj, 1.0, R, ws)
+        return R
+
     def ritz_block(roots, coef):
         """Ritz vectors X = V C and residuals R = HV C - X diag(e)."""
         with _measure(timers, "cegterg:upda", device):
@@ -775,6 +811,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
             dv.gemm(nat.OP_N, npw, number_of_roots, active, 1.0, HV[:active], coef, 0.0, R, ws)
             e = dv.upload(np.asarray(roots, dtype=np.float64), device)
             dv.subtract_band_energies(R, X, e, npw)
+            deflate(R)
         return X, R
 
     def selected_residuals(roots, coef, cols):
@@ -788,6 +825,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
             R = torch.empty((len(cols), npw), dtype=dtype, device=device)
             dv.gemm(nat.OP_N, npw, len(cols), active, 1.0, HV[:active], cu, 0.0, R, ws)
             dv.gemm(nat.OP_N, npw, len(cols), active, -1.0, V[:active], ce, 1.0, R, ws)
+            deflate(R)
         return R
 
     track = (residual_bounds is not None or residual_energy_scale is not None
@@ -840,11 +878,19 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
             # FP32 overlap near I that stays positive definite
             with _measure(timers, "cegterg:orth", device):
                 proj = torch.empty((len(cols), active), dtype=dtype, device=device)
+                if locked is not None:
+                    lproj = torch.empty((len(cols), locked.shape[0]), dtype=dtype, device=device)
                 for _ in range(2):
                     dv.gemm(nat.OP_C, active, len(cols), npw, 1.0, V[:active], corrections,
                             0.0, proj, ws)
                     dv.gemm(nat.OP_N, npw, len(cols), active, -1.0, V[:active], proj, 1.0,
                             corrections, ws)
+                    if locked is not None:
+                        nl = int(locked.shape[0])
+                        dv.gemm(nat.OP_C, nl, len(cols), npw, 1.0, locked, corrections, 0.0,
+                                lproj, ws)
+                        dv.gemm(nat.OP_N, npw, len(cols), nl, -1.0, locked, lproj, 1.0,
+                                corrections, ws)
                 block = _cholesky_orthonormalize(corrections, npw, ws)
                 if block is None:
                     block = _orthonormalize_device(corrections, npw, ws, threshold=1.0e-4)
@@ -887,11 +933,11 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
             X, R = ritz_block(values, coef)
             packed = False
             residual_norms = residual_norms_of(R)
-        converged = np.abs(values - previous) < energy_bounds
-        if residual_bounds is not None:
-            converged &= residual_norms < residual_bounds
+        converged = np.abs(values - previous) < eb
+        if rb is not None:
+            converged &= residual_norms < rb
         if residual_energy_scale is not None:
-            converged &= residual_norms ** 2 < residual_energy_scale * root_tolerances
+            converged &= residual_norms ** 2 < residual_energy_scale * rt
         unconverged = ~converged
         number_unconverged = int(np.count_nonzero(unconverged))
         if ITERATION_TRACE is not None:
@@ -902,6 +948,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
             if not track:
                 X, R = ritz_block(values, coef)
                 residual_norms = residual_norms_of(R)
+            values, X, residual_norms = merged(values, X, residual_norms)
             return _PhaseResult(values, X, residual_norms, iteration, True, 0, happ)
         if stall_window:
             worst = float(np.max(residual_norms[unconverged]))
@@ -912,15 +959,54 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
                     and min(progress[-stall_window:]) > 0.5 * min(progress[:-stall_window])):
                 break
         if active + number_unconverged > maximum_subspace:
+            if ITERATION_TRACE is not None:
+                ITERATION_TRACE.append(("restart", iteration, int(np.count_nonzero(converged)),
+                                        active, maximum_subspace))
             with _measure(timers, "cegterg:last", device):
                 if X is None:
                     X, R = ritz_block(values, coef)
                     packed = False
+                # lock the longest converged run from the bottom of the active
+                # roots that ends at a gap of at least MIXED_LOCK_GAP: the
+                # locked set is then a spectral subspace and no near-degenerate
+                # cluster is split between locked and active roots
+                run = int(np.argmin(converged)) if not converged.all() else len(converged)
+                while run > 0 and values[run] - values[run - 1] < MIXED_LOCK_GAP:
+                    run -= 1
+                done = np.arange(run)
+                if coarse and len(done) >= MIXED_LOCK_MIN:
+                    # lock the run; the search continues on the roots above it
+                    rest = np.arange(run, number_of_roots)
+                    di = dv.upload(done.astype(np.int64), device)
+                    ri = dv.upload(rest.astype(np.int64), device)
+                    rows_done = X.index_select(0, di)
+                    locked = rows_done if locked is None else torch.cat([locked, rows_done])
+                    lock_vals.append(values[done])
+                    lock_res.append(residual_norms[done] if track
+                                    else np.zeros(len(done)))
+                    X, R = X.index_select(0, ri), R.index_select(0, ri)
+                    values, residual_norms = values[rest], residual_norms[rest]
+                    gidx = gidx[rest]
+                    eb = energy_bounds[gidx]
+                    rb = None if residual_bounds is None else residual_bounds[gidx]
+                    rt = root_tolerances[gidx]
+                    number_of_roots = len(rest)
+                    unconverged = unconverged[rest]
+                    number_unconverged = int(np.count_nonzero(unconverged))
+                    # the search space keeps its size: a space shrunk with the
+                    # root count restarts too often for the slow top roots
+                    rr = DeviceRayleighRitz(maximum_subspace, number_of_roots, device, ws,
+                                            spectral=ctx.spectral)
+                    if ITERATION_TRACE is not None:
+                        ITERATION_TRACE.append(("lock", iteration, int(locked.shape[0]),
+                                                number_of_roots, maximum_subspace))
                 active = number_of_roots
                 V[:active].copy_(X)
                 e = dv.upload(values, device)
                 dv.restart_applied(X, R, e, HV[:active], npw)
                 rr.restart(values)
+                # the basis is the Ritz block itself
+                coef = torch.eye(number_of_roots, dtype=dtype, device=device)
         elif X is None:
             R = selected_residuals(values, coef, np.flatnonzero(unconverged))
             packed = True
@@ -928,6 +1014,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
         X, R = ritz_block(values, coef)
     if not track:
         residual_norms = residual_norms_of(R)
+    values, X, residual_norms = merged(values, X, residual_norms)
     return _PhaseResult(values, X, residual_norms, completed, False, number_unconverged, happ)
 
 
