@@ -18,7 +18,10 @@ pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).parent / "golden" / "batch_davidson.npz"
 TARGET_HA = 0.5e-8
 PROBLEMS = [(cell, k) for cell in ("cell54", "cell64", "cell72") for k in (0, 1)]
-KWARGS = dict(tolerance=0.1 * TARGET_HA, residual_factor=None, precision="mixed")
+# Davidson stops on |de| < tolerance, which bounds the error only up to the
+# convergence rate of the slowest root: as bench.py does, solve to a
+# hundredth of the target
+KWARGS = dict(tolerance=0.01 * TARGET_HA, residual_factor=None, precision="mixed")
 
 
 def device_ham(s):
