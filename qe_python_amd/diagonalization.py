@@ -67,6 +67,9 @@ MIXED_STALL_SPAN = 100.0
 # MIXED_LOCK_GAP) and continues on the others only
 MIXED_LOCK_MIN = 8
 MIXED_LOCK_GAP = 1.0e-3     # Ha
+# Gram-Schmidt passes of the complex64 phase's corrections against the basis
+# (one pass costs more iterations than the second pass saves)
+ORTHO_PASSES = 2
 # diagnostics: set to a list to receive one record per Davidson iteration
 # (storage dtype, iteration, unconverged roots, max |de|, max residual norm
 # or None when residuals are not formed)
@@ -871,16 +874,17 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
             corrections = torch.empty((len(cols), npw), dtype=dtype, device=device)
             dv.qe_precondition(R, rows_dev, e_dev, diag_dev, corrections, npw)
         if coarse and len(cols):
-            # complex64 phase: the block is made orthonormal to the basis (two
-            # classical Gram-Schmidt passes) and within itself (canonical
-            # orthonormalisation dropping directions below 1e-4 of the
-            # largest) -- the same span, so the same Rayleigh-Ritz, with an
-            # FP32 overlap near I that stays positive definite
+            # complex64 phase: the block is made orthogonal to the basis
+            # (classical Gram-Schmidt, twice) and orthonormal within itself
+            # (Cholesky QR, or canonical orthonormalisation dropping
+            # directions below 1e-4 of the largest) -- the same span, so the
+            # same Rayleigh-Ritz, with an FP32 overlap near I that stays
+            # positive definite
             with _measure(timers, "cegterg:orth", device):
                 proj = torch.empty((len(cols), active), dtype=dtype, device=device)
                 if locked is not None:
                     lproj = torch.empty((len(cols), locked.shape[0]), dtype=dtype, device=device)
-                for _ in range(2):
+                for _ in range(ORTHO_PASSES):
                     dv.gemm(nat.OP_C, active, len(cols), npw, 1.0, V[:active], corrections,
                             0.0, proj, ws)
                     dv.gemm(nat.OP_N, npw, len(cols), active, -1.0, V[:active], proj, 1.0,
