@@ -252,6 +252,98 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
   }
 }
 
+// z-stage kernels with register team FFTs for the lengths in QEPW_TEAM_SIZES:
+// one line per team (a workgroup holds TeamGeom<N>::teams sticks), the line
+// in LDS doubling as the team's exchange buffer; two workgroup barriers in
+// all instead of two per radix pass.
+template <typename T, int N>
+__global__ __launch_bounds__(kTeamThreads) void k_sticks_inverse_team(
+    const cplx<T>* __restrict__ psi, int64_t ldpsi, cplx<T>* __restrict__ work,
+    int64_t band_stride, int nst, const int* __restrict__ off,
+    const int* __restrict__ gpos, const cplx<T>* __restrict__ tw) {
+  using G = TeamGeom<N>;
+  constexpr int NP = N + 1;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* Ls = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int team_in_wave = lane / G::M;
+  const int t = lane - team_in_wave * G::M;
+  const int team = wave * G::per_wave + team_in_wave;
+  const int s0 = blockIdx.x * G::teams;
+  const int ns = min(G::teams, nst - s0);
+  const int64_t b = blockIdx.y;
+  const cplx<T> zero = mk<T>(T(0), T(0));
+  TeamTwiddles<T, N> twr;
+  twr.load(t, tw);
+  for (int i = tid; i < ns * NP; i += kTeamThreads) Ls[i] = zero;
+  __syncthreads();
+  const int d0 = off[s0], d1 = off[s0 + ns];
+  const int shift = s0 * N;
+  const cplx<T>* src = psi + b * ldpsi;
+  for (int d = d0 + tid; d < d1; d += kTeamThreads) {
+    const int q = gpos[d] - shift;  // l * N + z
+    Ls[q + q / N] = src[d];
+  }
+  __syncthreads();
+  if (team_in_wave < G::per_wave && team < ns)
+    team_fft_inplace<T, N, +1>(t, Ls + team * NP, twr);
+  __syncthreads();
+  cplx<T>* w = work + b * band_stride + s0;
+  for (int i = tid; i < ns * N; i += kTeamThreads) {
+    const int z = i / ns;
+    const int l = i - z * ns;
+    w[(int64_t)z * nst + l] = Ls[l * NP + z];
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(kTeamThreads) void k_sticks_forward_team(
+    const cplx<T>* __restrict__ work, int64_t band_stride, const cplx<T>* __restrict__ psi,
+    int64_t ldpsi, const T* __restrict__ kinetic, cplx<T>* __restrict__ out, int64_t ldout,
+    int nst, const int* __restrict__ off, const int* __restrict__ gpos,
+    const cplx<T>* __restrict__ tw, T inv_n) {
+  using G = TeamGeom<N>;
+  constexpr int NP = N + 1;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* Ls = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int team_in_wave = lane / G::M;
+  const int t = lane - team_in_wave * G::M;
+  const int team = wave * G::per_wave + team_in_wave;
+  const int s0 = blockIdx.x * G::teams;
+  const int ns = min(G::teams, nst - s0);
+  const int64_t b = blockIdx.y;
+  TeamTwiddles<T, N> twr;
+  twr.load(t, tw);
+  const cplx<T>* w = work + b * band_stride + s0;
+  for (int i = tid; i < ns * N; i += kTeamThreads) {
+    const int z = i / ns;
+    const int l = i - z * ns;
+    Ls[l * NP + z] = w[(int64_t)z * nst + l];
+  }
+  __syncthreads();
+  if (team_in_wave < G::per_wave && team < ns)
+    team_fft_inplace<T, N, -1>(t, Ls + team * NP, twr);
+  __syncthreads();
+  const int d0 = off[s0], d1 = off[s0 + ns];
+  const int shift = s0 * N;
+  const cplx<T>* src = psi + b * ldpsi;
+  cplx<T>* dst = out + b * ldout;
+  for (int d = d0 + tid; d < d1; d += kTeamThreads) {
+    const int q = gpos[d] - shift;
+    cplx<T> v = cscale(Ls[q + q / N], inv_n);
+    if (kinetic) {
+      const T k = kinetic[d];
+      const cplx<T> p = src[d];
+      v.x += k * p.x;
+      v.y += k * p.y;
+    }
+    dst[d] = v;
+  }
+}
+
 // Square plane sizes with an instantiated fast kernel.
 #define QEPW_TEAM_SIZES(X) \
   X(16) X(18) X(20) X(24) X(27) X(30) X(32) X(36) X(40) X(45) X(48) X(50) X(54) \
@@ -286,6 +378,31 @@ static bool launch_planes_team(const Plan& p, const StickTables& st, cplx<T>* wo
   }
   QEPW_TEAM_SIZES(QEPW_LAUNCH_TEAM)
 #undef QEPW_LAUNCH_TEAM
+  return false;
+}
+
+// Launches the team z-stage kernel for n3 if one exists; false otherwise.
+template <typename T>
+static bool launch_sticks_team(bool inverse, const Plan& p, const StickTables& st,
+                               const cplx<T>* psi, int64_t ldpsi, cplx<T>* work,
+                               int64_t bstride, const T* kinetic, cplx<T>* out, int64_t ldout,
+                               const cplx<T>* tw, T inv_n, int nb, hipStream_t stream) {
+#define QEPW_LAUNCH_STICKS(NN)                                                               \
+  if (p.n3 == NN) {                                                                          \
+    using G = TeamGeom<NN>;                                                                  \
+    const size_t lds = (size_t)G::teams * (NN + 1) * sizeof(cplx<T>);                        \
+    const dim3 grid((unsigned)((st.nst + G::teams - 1) / G::teams), (unsigned)nb);           \
+    if (inverse)                                                                             \
+      hipLaunchKernelGGL((k_sticks_inverse_team<T, NN>), grid, dim3(kTeamThreads), lds,      \
+                         stream, psi, ldpsi, work, bstride, st.nst, st.d_off, st.d_gpos, tw); \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_sticks_forward_team<T, NN>), grid, dim3(kTeamThreads), lds,      \
+                         stream, work, bstride, psi, ldpsi, kinetic, out, ldout, st.nst,     \
+                         st.d_off, st.d_gpos, tw, inv_n);                                    \
+    return true;                                                                             \
+  }
+  QEPW_TEAM_SIZES(QEPW_LAUNCH_STICKS)
+#undef QEPW_LAUNCH_STICKS
   return false;
 }
 
@@ -365,19 +482,24 @@ int apply_local_t(const Plan& p, const void* vectors, int64_t ldv, int nb,
     const cplx<T>* src = static_cast<const cplx<T>*>(vectors) + (int64_t)b0 * ldv;
     cplx<T>* dst = static_cast<cplx<T>*>(result) + (int64_t)b0 * ldr;
     dim3 gs(st.ngroups, nbc), gp(p.n3, nbc);
-    hipLaunchKernelGGL(k_sticks_inverse<T>, gs, dim3(kStickThreads), stick_lds, stream,
-                       src, ldv, work, bstride, st.nst, p.n3, st.d_off, st.d_gpos,
-                       st.group, p.line[2], twiddles<T>(p, 2));
+    if (!launch_sticks_team<T>(true, p, st, src, ldv, work, bstride, nullptr, nullptr, 0,
+                               twiddles<T>(p, 2), inv_n, nbc, stream))
+      hipLaunchKernelGGL(k_sticks_inverse<T>, gs, dim3(kStickThreads), stick_lds, stream,
+                         src, ldv, work, bstride, st.nst, p.n3, st.d_off, st.d_gpos,
+                         st.group, p.line[2], twiddles<T>(p, 2));
     if (!team || !launch_planes_team<T>(p, st, work, bstride, static_cast<const T*>(potential),
                                         twiddles<T>(p, 0), nbc, stream))
       hipLaunchKernelGGL((k_planes<T, kPlaneApply>), gp, dim3(kPlaneThreads), plane_lds,
                          stream, work, bstride, st.nst, st.d_plane, st.d_xcols, st.nxcols,
                          static_cast<const T*>(potential), static_cast<T*>(nullptr), p.n1,
                          p.n2, p.line[0], p.line[1], twiddles<T>(p, 0), twiddles<T>(p, 1));
-    hipLaunchKernelGGL(k_sticks_forward<T>, gs, dim3(kStickThreads), stick_lds, stream,
-                       work, bstride, src, ldv, static_cast<const T*>(kinetic), dst, ldr,
-                       st.nst, p.n3, st.d_off, st.d_gpos, st.group, p.line[2],
-                       twiddles<T>(p, 2), inv_n);
+    if (!launch_sticks_team<T>(false, p, st, src, ldv, work, bstride,
+                               static_cast<const T*>(kinetic), dst, ldr, twiddles<T>(p, 2),
+                               inv_n, nbc, stream))
+      hipLaunchKernelGGL(k_sticks_forward<T>, gs, dim3(kStickThreads), stick_lds, stream,
+                         work, bstride, src, ldv, static_cast<const T*>(kinetic), dst, ldr,
+                         st.nst, p.n3, st.d_off, st.d_gpos, st.group, p.line[2],
+                         twiddles<T>(p, 2), inv_n);
   }
   return hipGetLastError() == hipSuccess ? QEPW_OK : QEPW_ERR_HIP;
 }

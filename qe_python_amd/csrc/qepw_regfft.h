@@ -195,4 +195,47 @@ __device__ __forceinline__ void team_fft(int t, cplx<T>* scratch, const TeamTwid
   wave_lds_sync();
 }
 
+// team_fft on a line held in LDS, with the line itself as the exchange
+// buffer (every input is in registers before the first write): line[k] is
+// replaced by X[k], k < N.  The team lives in one wave, so wave-level syncs
+// order its LDS accesses.
+template <typename T, int N, int SIGN>
+__device__ __forceinline__ void team_fft_inplace(int t, cplx<T>* line,
+                                                 const TeamTwiddles<T, N>& tw) {
+  constexpr int L = team_L(N);
+  constexpr int M = N / L;
+  cplx<T> a[M];
+  if (t < L) {
+#pragma unroll
+    for (int j2 = 0; j2 < M; ++j2) a[j2] = line[t + L * j2];
+    RegDft<T, M, SIGN>::run(a);
+#pragma unroll
+    for (int k1 = 1; k1 < M; ++k1) {
+      if (t > 0) {
+        cplx<T> w = tw.w[k1];
+        if (SIGN > 0) w.y = -w.y;
+        a[k1] = cmul(a[k1], w);
+      }
+    }
+  }
+  wave_lds_sync();
+  if (t < L) {
+#pragma unroll
+    for (int k1 = 0; k1 < M; ++k1) line[t * M + k1] = a[k1];
+  }
+  wave_lds_sync();
+  cplx<T> b[L];
+  if (t < M) {
+#pragma unroll
+    for (int j1 = 0; j1 < L; ++j1) b[j1] = line[j1 * M + t];
+    RegDft<T, L, SIGN>::run(b);
+  }
+  wave_lds_sync();
+  if (t < M) {
+#pragma unroll
+    for (int k2 = 0; k2 < L; ++k2) line[t + M * k2] = b[k2];
+  }
+  wave_lds_sync();
+}
+
 }  // namespace qepw
