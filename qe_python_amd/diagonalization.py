@@ -680,7 +680,8 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
     coarse.X = None
     fine = _davidson_phase(
         ctx, start, torch.complex128, energy_bounds, residual_bounds, residual_energy_scale,
-        root_tolerances, max_iterations, False, True, None, lambda active: None, perm_dev)
+        root_tolerances, max_iterations, False, True, None, lambda active: None, perm_dev,
+        seed_spectrum=coarse.spectrum)
     return DavidsonResult(fine.values, _result_vectors(fine.X, perm_dev, npw, device_output),
                           fine.residual_norms, coarse.iterations + fine.iterations,
                           fine.converged, fine.number_unconverged, coarse.happ + fine.happ)
@@ -709,6 +710,7 @@ class _PhaseResult:
     converged: bool
     number_unconverged: int
     happ: int
+    spectrum: np.ndarray | None = None   # eigenvalue estimates, also above the roots
 
 
 def _davidson_phase(ctx, *args, **kwargs):
@@ -719,7 +721,7 @@ def _davidson_phase(ctx, *args, **kwargs):
 def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, residual_energy_scale,
                          root_tolerances, max_iterations, start_is_ritz, start_is_orthonormal,
                          initial_eigenvalues, applied_block, perm_dev, *, stall_window=None,
-                         require_residuals=False, coarse=False):
+                         require_residuals=False, coarse=False, seed_spectrum=None):
     """The restarted Davidson loop (``diagonalization.py:640-880``) at one
     storage dtype, from a device-order start block ``start`` (nb, npw).
 
@@ -746,6 +748,11 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
     eb, rb, rt = energy_bounds, residual_bounds, root_tolerances
     locked = None                 # locked Ritz vectors (device rows)
     lock_vals, lock_res = [], []
+
+    def spectrum():
+        """Eigenvalue estimates of the phase, locked ones included."""
+        top = rr.theta if rr.theta is not None else values
+        return np.sort(np.concatenate(lock_vals + [np.asarray(top, dtype=np.float64)]))
 
     def merged(values, X, residual_norms):
         """Locked and active Ritz pairs together, by ascending value."""
@@ -849,6 +856,11 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
     else:
         values, coef = ritz_solve()
         X, R = ritz_block(values, coef)
+        if seed_spectrum is not None and rr.theta is not None:
+            # estimates from an earlier stage above the roots: the next solve
+            # can place its shift instead of diagonalising densely
+            known = len(rr.theta)
+            rr.seed(np.concatenate([rr.theta, np.asarray(seed_spectrum)[known:]]))
 
     def residual_norms_of(block):
         sq = dv.column_squared_norms(block, npw).cpu().numpy()
@@ -952,8 +964,9 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
             if not track:
                 X, R = ritz_block(values, coef)
                 residual_norms = residual_norms_of(R)
+            est = spectrum()
             values, X, residual_norms = merged(values, X, residual_norms)
-            return _PhaseResult(values, X, residual_norms, iteration, True, 0, happ)
+            return _PhaseResult(values, X, residual_norms, iteration, True, 0, happ, est)
         if stall_window:
             worst = float(np.max(residual_norms[unconverged]))
             progress.append(worst)
@@ -978,6 +991,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
                 while run > 0 and values[run] - values[run - 1] < MIXED_LOCK_GAP:
                     run -= 1
                 done = np.arange(run)
+                old_theta = None
                 if coarse and len(done) >= MIXED_LOCK_MIN:
                     # lock the run; the search continues on the roots above it
                     rest = np.arange(run, number_of_roots)
@@ -999,6 +1013,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
                     number_unconverged = int(np.count_nonzero(unconverged))
                     # the search space keeps its size: a space shrunk with the
                     # root count restarts too often for the slow top roots
+                    old_theta = rr.theta
                     rr = DeviceRayleighRitz(maximum_subspace, number_of_roots, device, ws,
                                             spectral=ctx.spectral)
                     if ITERATION_TRACE is not None:
@@ -1009,6 +1024,10 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
                 e = dv.upload(values, device)
                 dv.restart_applied(X, R, e, HV[:active], npw)
                 rr.restart(values)
+                if len(lock_vals) and rr.theta is None and old_theta is not None:
+                    # the fresh pencil after locking: the previous estimates
+                    # above the locked run
+                    rr.seed(np.asarray(old_theta)[len(done):])
                 # the basis is the Ritz block itself
                 coef = torch.eye(number_of_roots, dtype=dtype, device=device)
         elif X is None:
@@ -1018,8 +1037,10 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
         X, R = ritz_block(values, coef)
     if not track:
         residual_norms = residual_norms_of(R)
+    est = spectrum()
     values, X, residual_norms = merged(values, X, residual_norms)
-    return _PhaseResult(values, X, residual_norms, completed, False, number_unconverged, happ)
+    return _PhaseResult(values, X, residual_norms, completed, False, number_unconverged, happ,
+                        est)
 
 
 def _result_vectors(X, perm_dev, npw, device_output=False):

@@ -97,6 +97,8 @@ class DeviceRayleighRitz:
         self.standard = 0            # W and A are valid for [:standard]
         self.theta = None            # previous Ritz values (host, ascending)
         self.sigma = None            # shift of the previous spectral solve
+        self.drift = None            # max change of the lowest k Ritz values between solves
+        self._last_k = None
         self._bound_prev = None      # ||A - sigma I||_1 of the previous solve (host)
         self.u_prev = None           # previous lowest eigenvectors of A (device, (k, maxsub))
         self.stats = {"spectral": 0, "dense": 0, "host": 0, "ns_iterations": 0,
@@ -146,12 +148,24 @@ class DeviceRayleighRitz:
             nat.check(lib.qepw_set_diagonal(mat.data_ptr(), self.maxsub, k, d, st),
                       "qepw_set_diagonal")
         self.standard = k
+        # the shift stays valid: the new basis holds the k Ritz vectors, all
+        # below it, so by interlacing at least k eigenvalues stay below it.
+        # The restart keeps the Ritz values: the next solve's drift is
+        # measured against them
+        self._last_k = np.asarray(values[:self.nroots], dtype=np.float64).copy()
+        # the Ritz vectors are the new basis: the eigenvectors of A = diag
+        if self.u_prev is None:
+            self.u_prev = torch.zeros((self.nroots, self.maxsub), dtype=C128, device=self.device)
+        self.u_prev.zero_()
+        idx = torch.arange(min(k, self.u_prev.shape[0]), device=self.device)
+        self.u_prev[idx, idx] = 1.0
+
+    def seed(self, values: np.ndarray):
+        """Ritz-value estimates (ascending) of the pencil to come, e.g. those
+        a previous stage found above the roots: they let the first solve after
+        a restart place its shift instead of diagonalising densely."""
+        self.theta = np.sort(np.asarray(values, dtype=np.float64))
         self.sigma = None
-        if self.u_prev is not None:
-            # the Ritz vectors are the new basis: the eigenvectors of A = diag
-            self.u_prev.zero_()
-            idx = torch.arange(min(k, self.u_prev.shape[0]), device=self.device)
-            self.u_prev[idx, idx] = 1.0
 
     # -- step 2: standardisation ------------------------------------------
     def _standardize(self, active: int):
@@ -211,7 +225,8 @@ class DeviceRayleighRitz:
         with _one_blas_thread():
             self._standardize(active)
             out = None
-            if self.theta is not None and len(self.theta) >= k + 2 and active > k + 8:
+            if self.theta is not None and active > k + 8 and (
+                    len(self.theta) >= k + 2 or self._can_keep_shift()):
                 out = self._spectral(active)
             else:
                 self.stats["skipped"] += 1
@@ -264,9 +279,23 @@ class DeviceRayleighRitz:
         self._gemm(nat.OP_C, n, k, n, 1.0, self.block(self.W, 0, n, 0, n), U, 0.0, coef)
         self._keep_vectors(U, n)
         values = w.cpu().numpy().astype(np.float64)
+        self._note_drift(values)
         self.theta = values
         self.sigma = None
         return values[:k].copy(), coef
+
+    def _can_keep_shift(self) -> bool:
+        theta, k = self.theta, self.nroots
+        return (self.sigma is not None and theta is not None
+                and k <= len(theta) <= k + self.extra and theta[-1] < self.sigma
+                and self.drift is not None and self.drift < 0.1 * (self.sigma - theta[-1]))
+
+    def _note_drift(self, values):
+        k = self.nroots
+        low = np.asarray(values[:k], dtype=np.float64)
+        if self._last_k is not None and len(self._last_k) == len(low):
+            self.drift = float(np.max(np.abs(low - self._last_k)))
+        self._last_k = low.copy()
 
     def _keep_vectors(self, U, n):
         k = self.nroots
@@ -280,11 +309,12 @@ class DeviceRayleighRitz:
         st = dv.stream_handle(self.device)
         lib = nat.load()
         theta = self.theta
-        if (self.sigma is not None and k + 4 <= len(theta) <= k + self.extra
-                and theta[-1] < self.sigma):
+        if self._can_keep_shift():
             # keep the previous shift: the subspace only grew since, so by
-            # Cauchy interlacing at least the len(theta) >= k + 4 eigenvalues
-            # found below it last time still lie below it
+            # Cauchy interlacing at least the len(theta) >= k eigenvalues
+            # found below it last time still lie below it; while the Ritz
+            # values still drop fast (drift comparable to the distance to the
+            # shift) many more would, so a shift from the gaps is taken then
             sigma = self.sigma
             lo_v, hi_v = theta[-1], 2.0 * sigma - theta[-1]
         else:
@@ -410,6 +440,7 @@ class DeviceRayleighRitz:
         coef = torch.empty((k, n), dtype=C128, device=self.device)
         self._gemm(nat.OP_C, n, k, n, 1.0, self.block(self.W, 0, n, 0, n), U, 0.0, coef)
         self._keep_vectors(U, n)
+        self._note_drift(w)
         self.theta = np.asarray(w, dtype=np.float64)
         self.sigma = sigma
         return w[:k].copy(), coef
