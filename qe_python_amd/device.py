@@ -114,11 +114,23 @@ class Workspace:
 
 def gemm(op_a: int, m: int, n: int, k: int, alpha, a: torch.Tensor, b: torch.Tensor,
          beta, c: torch.Tensor, ws: Workspace) -> None:
-    """c = alpha op(a) b + beta c on the device (column-major views)."""
+    """c = alpha op(a) b + beta c on the device (column-major views).
+
+    Every operand view must hold the matrix the sizes name (a view of
+    ``ncols`` columns of ``nrows`` elements is a ``(>= ncols, >= nrows)``
+    tensor), so that no size slip reaches the device as an access past an
+    allocation."""
     lib = nat.load()
     prec = precision_code(c.dtype)
     if a.dtype != c.dtype or b.dtype != c.dtype:
         raise ValueError("gemm operands must share one complex dtype")
+    if min(m, n, k) > 0:
+        a_shape = (m, k) if op_a & nat.OP_C else (k, m)
+        b_shape = (k, n) if op_a & nat.OP_BC else (n, k)
+        for name, t, (cols, rows) in (("a", a, a_shape), ("b", b, b_shape), ("c", c, (n, m))):
+            if t.dim() != 2 or t.shape[0] < cols or t.shape[1] < rows:
+                raise ValueError(f"gemm operand {name} of shape {tuple(t.shape)} is smaller "
+                                 f"than the {rows} x {cols} matrix the sizes name")
     need = lib.qepw_gemm_workspace(prec, op_a, m, n, k)
     buf = ws.get(need)
     rc = lib.qepw_gemm(prec, op_a, m, n, k, nat.complex_pair(alpha), a.data_ptr(), ld_of(a),

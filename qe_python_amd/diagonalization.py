@@ -481,6 +481,18 @@ class _HostOperator:
         dst.copy_(torch.from_numpy(np.ascontiguousarray(out.T)).to(dst.dtype))
 
 
+def _check_room(active: int, count: int, maximum_subspace: int) -> None:
+    """The reference stores the correction block in its fixed
+    ``basis_storage`` (``qepy_pw/diagonalization.py:789-791``), which raises
+    ``ValueError`` when an initial basis of more than ``maximum_subspace -
+    unconverged`` vectors leaves no room for it; after a restart the
+    ``active + unconverged`` test always leaves room."""
+    if active + count > maximum_subspace:
+        raise ValueError(f"Davidson subspace of {maximum_subspace} vectors has no room for "
+                         f"{count} corrections after {active} basis vectors; pass fewer "
+                         f"initial vectors or a larger subspace_multiplier")
+
+
 def _cholesky_orthonormalize(block: torch.Tensor, npw: int, ws, condition: float = 1.0e6):
     """Cholesky QR of a block of complex64 corrections: Gram = L L^H on the
     host (one thread), block L^-H on the device.  None when the Gram matrix
@@ -912,6 +924,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
                     block = _orthonormalize_device(corrections, npw, ws, threshold=1.0e-4)
             if not block.shape[0]:
                 break
+            _check_room(active, int(block.shape[0]), maximum_subspace)
             old = active
             active += int(block.shape[0])
             V[old:active].copy_(block)
@@ -927,6 +940,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
             keep = np.flatnonzero(sq > (1.0e-14) ** 2 * scale).astype(np.int32)
             if not len(keep):
                 break
+            _check_room(active, len(keep), maximum_subspace)
             old = active
             active += len(keep)
             dv.normalize_selected_columns(corrections, sq_dev, dv.upload(keep, device),

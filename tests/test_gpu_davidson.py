@@ -106,6 +106,12 @@ def test_errors_and_nonconvergence(small):
     dep[:, 0] = s.trial[:, 0]
     with pytest.raises(ValueError, match="linearly dependent"):
         davidson(ham.apply, ham.diagonal, 2, initial_vectors=dep)
+    # an initial basis that leaves the subspace no room for the first
+    # correction block (16 trial vectors, 4 roots, 16-vector subspace)
+    for precision in ("fp64", "mixed"):
+        with pytest.raises(ValueError, match="no room"):
+            davidson(ham.apply, ham.diagonal, 4, initial_vectors=s.trial[:, :16],
+                     tolerance=TOL_HA, precision=precision)
     res = davidson(ham.apply, ham.diagonal, 7, initial_vectors=s.trial, max_iterations=1,
                    tolerance=TOL_HA)
     assert not res.converged and res.iterations == 1 and res.number_unconverged > 0

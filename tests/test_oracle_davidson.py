@@ -108,6 +108,13 @@ def test_argument_errors():
     dep[:, 0] = 1.0
     with pytest.raises(ValueError, match="linearly dependent"):
         odav.davidson(op, d, 2, initial_vectors=dep)
+    # 14 initial vectors for 4 roots fill 14 of the 16-vector subspace: the
+    # first 4-column correction block has no room (the reference's
+    # basis_storage assignment, qepy_pw/diagonalization.py:791, raises)
+    start = np.eye(18, 14, dtype=complex) + 0.01
+    with pytest.raises(ValueError):
+        odav.davidson(op, d, 4, initial_vectors=start)
+    assert odav.davidson(op, d, 4, initial_vectors=start[:, :12]).iterations >= 1
 
 
 def test_non_convergence_is_a_result_not_an_error():
