@@ -145,10 +145,13 @@ __global__ __launch_bounds__(kPlaneThreads) void k_planes(
 }
 
 // Fast xy-plane kernel for square planes (n1 = n2 = N): the LDS holds only
-// the nxc active x columns of the plane, Q[c * N + y]; every 1-D FFT is a
-// register "team" FFT (qepw_regfft.h) and the row pass fuses the backward
-// x-FFT, the V(r) multiply and the forward x-FFT without leaving registers
-// except for the team exchanges.  Four workgroup barriers per plane.
+// the nxc active x columns of the plane, Q[c * NP + y], plus one all-zero
+// column; every 1-D FFT is a register "team" FFT (qepw_regfft.h).  The row
+// pass chains the backward x-FFT (DIT), the V(r) multiply and the forward
+// x-FFT (DIF) in registers -- the DIF consumes the DIT's output order and
+// returns the DIT's input order -- so a row costs two team exchanges; the
+// column passes alternate DIT and DIF so that a team's consecutive lines need
+// one wave sync each.  Four workgroup barriers per plane.
 constexpr int kTeamThreads = 256;
 
 template <int N> struct TeamGeom {
@@ -158,41 +161,79 @@ template <int N> struct TeamGeom {
   static constexpr int teams = (kTeamThreads / 64) * per_wave;
 };
 
+// One y-column of the plane, in place: DIT for even rounds, DIF for odd ones.
+template <typename T, int N, int SIGN>
+__device__ __forceinline__ void team_column(int t, bool dif, cplx<T>* col, cplx<T>* S,
+                                            const TeamTwiddles<T, N>& tw) {
+  constexpr int L = team_L(N);
+  constexpr int M = N / L;
+  cplx<T> a[M], b[L];
+  if (!dif) {
+    if (t < L) {
+#pragma unroll
+      for (int s = 0; s < M; ++s) a[s] = col[t + L * s];
+    }
+    dit_front<T, N, SIGN>(t, S, tw, a);
+    wave_lds_sync();
+    dit_back<T, N, SIGN>(t, S, b);
+    if (t < M) {
+#pragma unroll
+      for (int k = 0; k < L; ++k) col[t + M * k] = b[k];
+    }
+  } else {
+    if (t < M) {
+#pragma unroll
+      for (int s = 0; s < L; ++s) b[s] = col[t + M * s];
+    }
+    dif_front<T, N, SIGN>(t, S, tw, b);
+    wave_lds_sync();
+    dif_back<T, N, SIGN>(t, S, a);
+    if (t < L) {
+#pragma unroll
+      for (int k = 0; k < M; ++k) col[t + L * k] = a[k];
+    }
+  }
+}
+
 template <typename T, int N>
 __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
     cplx<T>* __restrict__ work, int64_t band_stride, int nst,
     const int* __restrict__ stick_q, const int* __restrict__ colmap, int nxc,
     const T* __restrict__ V, const cplx<T>* __restrict__ tw) {
   using G = TeamGeom<N>;
+  constexpr int L = G::L, M = G::M;
   extern __shared__ unsigned char smem_raw[];
   // Q[c * NP + y]: active column c, padded stride NP = N + 1 so that the
-  // columns (and the teams' scratch lines) start on different LDS banks
+  // columns (and the teams' scratch lines) start on different LDS banks.
+  // Column nxc stays zero through the column passes: the row pass reads the
+  // inactive x positions from it and writes their (discarded) outputs to it.
   constexpr int NP = N + 1;
   cplx<T>* Q = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* scratch_all = Q + nxc * NP;
+  cplx<T>* scratch_all = Q + (nxc + 1) * NP;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int team_in_wave = lane / G::M;
-  const int t = lane - team_in_wave * G::M;
+  const int team_in_wave = lane / M;
+  const int t = lane - team_in_wave * M;
   const bool in_team = team_in_wave < G::per_wave;
   const int team = wave * G::per_wave + team_in_wave;
   cplx<T>* S = scratch_all + team * NP;
   const int z = blockIdx.x;
   const int64_t b = blockIdx.y;
-  const int nq = nxc * NP;
+  const int nq = (nxc + 1) * NP;
   const cplx<T> zero = mk<T>(T(0), T(0));
   cplx<T>* w = work + b * band_stride + (int64_t)z * nst;
 
-  // per-lane tables, the same for every line of the plane: twiddles, and
-  // the active-column slots of this lane's row inputs j = t + L s and row
-  // outputs k = t + M s (-1: inactive x column)
+  // per-lane tables, the same for every row of the plane: twiddles, and the
+  // Q offsets of this lane's row inputs x = t + L s (lanes t < L), which are
+  // also the row outputs it produces
   TeamTwiddles<T, N> twr;
   twr.load(t, tw);
-  int cin[G::M], cout[G::L];
+  int cq[M];
 #pragma unroll
-  for (int s = 0; s < G::M; ++s) cin[s] = t < G::L ? colmap[t + G::L * s] : -1;
-#pragma unroll
-  for (int s = 0; s < G::L; ++s) cout[s] = t < G::M ? colmap[t + G::M * s] : -1;
+  for (int s = 0; s < M; ++s) {
+    const int c = t < L ? colmap[t + L * s] : -1;
+    cq[s] = (c < 0 ? nxc : c) * NP;
+  }
 
   for (int i = tid; i < nq; i += kTeamThreads) Q[i] = zero;
   __syncthreads();
@@ -204,11 +245,9 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
 
   // backward y-FFT of every active column (in place in Q)
   if (in_team) {
-    for (int c = team; c < nxc; c += G::teams) {
-      cplx<T>* col = Q + c * NP;
-      team_fft<T, N, +1>(t, S, twr, [&](int j, int) { return col[j]; },
-                         [&](int k, int, cplx<T> v) { col[k] = v; });
-    }
+    bool dif = false;
+    for (int c = team; c < nxc; c += G::teams, dif = !dif)
+      team_column<T, N, +1>(t, dif, Q + c * NP, S, twr);
   }
   __syncthreads();
 
@@ -217,33 +256,38 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
   if (in_team) {
     const T* vz = V + (int64_t)z * N * N;
     for (int y = team; y < N; y += G::teams) {
-      // V(r) of this lane's outputs, fetched before the FFT needs them
-      T vv[G::L];
+      // V(r) at this lane's backward outputs x = t + M k, fetched first
+      T vv[L];
 #pragma unroll
-      for (int s = 0; s < G::L; ++s) vv[s] = t < G::M ? vz[y * N + t + G::M * s] : T(0);
-      team_fft<T, N, +1>(
-          t, S, twr,
-          [&](int, int s) { return cin[s] >= 0 ? Q[cin[s] * NP + y] : zero; },
-          [&](int k, int s, cplx<T> v) { S[k] = cscale(v, vv[s]); });
-      // S now holds V psi of this row in natural order (written after the
-      // team's last exchange read, read back before any exchange write)
+      for (int k = 0; k < L; ++k) vv[k] = t < M ? vz[y * N + t + M * k] : T(0);
+      cplx<T> a[M], bb[L];
+      if (t < L) {
+#pragma unroll
+        for (int s = 0; s < M; ++s) a[s] = Q[cq[s] + y];
+      }
+      dit_front<T, N, +1>(t, S, twr, a);
       wave_lds_sync();
-      team_fft<T, N, -1>(
-          t, S, twr, [&](int j, int) { return S[j]; },
-          [&](int, int s, cplx<T> v) {
-            if (cout[s] >= 0) Q[cout[s] * NP + y] = v;
-          });
+      dit_back<T, N, +1>(t, S, bb);
+      if (t < M) {
+#pragma unroll
+        for (int k = 0; k < L; ++k) bb[k] = cscale(bb[k], vv[k]);
+      }
+      dif_front<T, N, -1>(t, S, twr, bb);
+      wave_lds_sync();
+      dif_back<T, N, -1>(t, S, a);
+      if (t < L) {
+#pragma unroll
+        for (int s = 0; s < M; ++s) Q[cq[s] + y] = a[s];
+      }
     }
   }
   __syncthreads();
 
   // forward y-FFT of every active column
   if (in_team) {
-    for (int c = team; c < nxc; c += G::teams) {
-      cplx<T>* col = Q + c * NP;
-      team_fft<T, N, -1>(t, S, twr, [&](int j, int) { return col[j]; },
-                         [&](int k, int, cplx<T> v) { col[k] = v; });
-    }
+    bool dif = false;
+    for (int c = team; c < nxc; c += G::teams, dif = !dif)
+      team_column<T, N, -1>(t, dif, Q + c * NP, S, twr);
   }
   __syncthreads();
   for (int s = tid; s < nst; s += kTeamThreads) {
@@ -355,7 +399,8 @@ static size_t team_lds_bytes(int n, int nxc) {
 #define QEPW_TEAMS_OF(NN) if (n == NN) teams = TeamGeom<NN>::teams;
   QEPW_TEAM_SIZES(QEPW_TEAMS_OF)
 #undef QEPW_TEAMS_OF
-  return ((size_t)nxc + teams) * (size_t)(n + 1) * sizeof(cplx<T>);
+  // active columns, the zero column of k_planes_team, one scratch line per team
+  return ((size_t)nxc + 1 + teams) * (size_t)(n + 1) * sizeof(cplx<T>);
 }
 
 // Launches the fast plane kernel if one exists for this plane; returns false

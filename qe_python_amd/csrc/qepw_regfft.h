@@ -142,18 +142,95 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Per-lane twiddles of a team FFT: w[k1] = w_N^(t k1), k1 < M, for the
-// lane's index t < L in its team.  They are the same for every line of a
-// plane, so a kernel loads them into registers once.
+// Per-lane twiddles of a team FFT, w[k] = w_N^(t k) for the lane's index t
+// in its team: k < M for t < L (the decimation-in-time split below) and
+// k < L for t < M (the decimation-in-frequency split); the largest index,
+// (L-1)(M-1), stays inside the N-entry table.  They are the same for every
+// line of a plane, so a kernel loads them into registers once.
 template <typename T, int N> struct TeamTwiddles {
   static constexpr int L = team_L(N);
   static constexpr int M = N / L;
   cplx<T> w[M];
   __device__ __forceinline__ void load(int t, const cplx<T>* __restrict__ tw) {
 #pragma unroll
-    for (int k1 = 0; k1 < M; ++k1) w[k1] = t < L ? tw[t * k1] : mk<T>(T(1), T(0));
+    for (int k = 0; k < M; ++k)
+      w[k] = (t < M && (t < L || k < L)) ? tw[t * k] : mk<T>(T(1), T(0));
   }
 };
+
+// ---- team FFT halves ------------------------------------------------------
+// One N = L*M point line by the M lanes of a team, split at its exchange so
+// that callers can chain transforms in registers.  Two factorisations:
+//
+//   DIT: lanes t < L hold inputs x[t + L s] (s < M);  dit_front takes the
+//        M-point DFTs and twiddles and writes S[t*M + k];  after a wave sync
+//        dit_back (lanes t < M) reads S[j*M + t] (j < L) and leaves
+//        X[t + M k] (k < L) in the lane's registers.
+//   DIF: lanes t < M hold inputs x[t + M s] (s < L) -- the DIT's output
+//        order;  dif_front takes the L-point DFTs and twiddles and writes
+//        S[k*M + t];  after a wave sync dif_back (lanes t < L) reads
+//        S[t*M + j] (j < M) and leaves X[t + L k] (k < M) -- the DIT's
+//        input order.
+//
+// Every lane's exchange writes land on exactly the slots that the same lane
+// read in the other factorisation's back half, so a DIT and a DIF may follow
+// each other (in either order) on one scratch line with no wave sync between
+// them; two DITs (or two DIFs) in a row need one.  Twiddles by w^0 = 1 are
+// exact, so they are applied to every lane without a branch.
+template <typename T, int N, int SIGN>
+__device__ __forceinline__ void dit_front(int t, cplx<T>* S, const TeamTwiddles<T, N>& tw,
+                                          cplx<T> (&a)[N / team_L(N)]) {
+  constexpr int L = team_L(N);
+  constexpr int M = N / L;
+  if (t < L) {
+    RegDft<T, M, SIGN>::run(a);
+#pragma unroll
+    for (int k = 1; k < M; ++k) {
+      a[k] = SIGN > 0 ? cmulc(a[k], tw.w[k]) : cmul(a[k], tw.w[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) S[t * M + k] = a[k];
+  }
+}
+
+template <typename T, int N, int SIGN>
+__device__ __forceinline__ void dit_back(int t, const cplx<T>* S, cplx<T> (&b)[team_L(N)]) {
+  constexpr int L = team_L(N);
+  constexpr int M = N / L;
+  if (t < M) {
+#pragma unroll
+    for (int j = 0; j < L; ++j) b[j] = S[j * M + t];
+    RegDft<T, L, SIGN>::run(b);
+  }
+}
+
+template <typename T, int N, int SIGN>
+__device__ __forceinline__ void dif_front(int t, cplx<T>* S, const TeamTwiddles<T, N>& tw,
+                                          cplx<T> (&b)[team_L(N)]) {
+  constexpr int L = team_L(N);
+  constexpr int M = N / L;
+  if (t < M) {
+    RegDft<T, L, SIGN>::run(b);
+#pragma unroll
+    for (int k = 1; k < L; ++k) {
+      b[k] = SIGN > 0 ? cmulc(b[k], tw.w[k]) : cmul(b[k], tw.w[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < L; ++k) S[k * M + t] = b[k];
+  }
+}
+
+template <typename T, int N, int SIGN>
+__device__ __forceinline__ void dif_back(int t, const cplx<T>* S,
+                                         cplx<T> (&a)[N / team_L(N)]) {
+  constexpr int L = team_L(N);
+  constexpr int M = N / L;
+  if (t < L) {
+#pragma unroll
+    for (int j = 0; j < M; ++j) a[j] = S[t * M + j];
+    RegDft<T, M, SIGN>::run(a);
+  }
+}
 
 // One N-point line by the M lanes of a team; ``t`` is the lane's index in
 // the team; ``scratch`` holds N elements private to the team; ``tw`` holds
