@@ -262,9 +262,12 @@ int qepw_projector_forces(int precision, const void* dO, int64_t lddo, int64_t d
  * qepw_set_diagonal: A[:n, :n] = diag(d), identity when d is NULL.
  * qepw_sign_iteration: the scaled Newton-Schulz steps
  *   X <- (a/2) X (3 I - a^2 X^2), one per entry of alphas[0:nalpha], on the
- *   n x n complex128 X held in x (x2 and y: scratch of the same shape and
- *   leading dimension); the result is left in x2 when *in_x2 is set, else
- *   in x.  Workspace: qepw_sign_iteration_workspace(n) bytes.
+ *   n x n complex128 X held in x, which must be exactly Hermitian (as
+ *   qepw_shift_scale leaves it); x2 and y: scratch of the same shape and
+ *   leading dimension.  Every iterate is exactly Hermitian; the result is
+ *   left in x2 when *in_x2 is set, else in x, and y is left holding its
+ *   square (the input of qepw_sign_check).  Workspace:
+ *   qepw_sign_iteration_workspace(n) bytes (none at present).
  */
 int qepw_hermitian_complete(void* a, int64_t lda, int first, int active, void* stream);
 int qepw_shift_scale(const void* a, int64_t lda, int n, double sigma, void* x, int64_t ldx,

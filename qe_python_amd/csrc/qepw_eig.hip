@@ -201,42 +201,30 @@ int sign_check(const void* x, int64_t ldx, const void* x2, int64_t ld2, int n, d
   return status();
 }
 
-size_t sign_iteration_workspace(int n) {
-  return gemm_workspace_bytes(QEPW_C128, QEPW_OP_N, n, n, n);
-}
+size_t sign_iteration_workspace(int) { return 0; }
 
 // Scaled Newton-Schulz steps X <- (a/2) X (3 I - a^2 X^2), one per entry of
-// ``alphas``: Y = X X, X' = 1.5 a X - 0.5 a^3 X Y (the GEMM accumulates onto
-// a copy of X), re-symmetrised every fourth step and after the last.  The
-// iterate ping-pongs between x and x2; *in_x2 tells which holds the result.
+// ``alphas``: Y = X X, X' = 1.5 a X - 0.5 a^3 X Y, each a Hermitian product
+// (herm_product: lower-triangle tiles, mirrored, so every iterate is exactly
+// Hermitian).  The iterate ping-pongs between x and x2; *in_x2 tells which
+// holds the result, and y is left holding its square for the caller's
+// convergence check.
 int sign_iteration(void* x, void* x2, void* y, int64_t ld, int n, const double* alphas,
-                   int nalpha, int* in_x2, void* ws, size_t ws_bytes, hipStream_t stream) {
+                   int nalpha, int* in_x2, void*, size_t, hipStream_t stream) {
   double2* cur = static_cast<double2*>(x);
   double2* nxt = static_cast<double2*>(x2);
-  const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
-  const size_t pitch = (size_t)ld * sizeof(double2);
   for (int i = 0; i < nalpha; ++i) {
     const double a = alphas[i];
-    int rc = gemm(QEPW_C128, QEPW_OP_N, n, n, n, one, cur, ld, cur, ld, zero, y, ld, ws,
-                  ws_bytes, stream);
+    int rc = herm_product(n, cur, cur, cur, y, ld, 1.0, 0.0, stream);
     if (rc) return rc;
-    if (hipMemcpy2DAsync(nxt, pitch, cur, pitch, (size_t)n * sizeof(double2), (size_t)n,
-                         hipMemcpyDeviceToDevice, stream) != hipSuccess)
-      return QEPW_ERR_HIP;
-    const double al[2] = {-0.5 * a * a * a, 0.0}, be[2] = {1.5 * a, 0.0};
-    rc = gemm(QEPW_C128, QEPW_OP_N, n, n, n, al, cur, ld, y, ld, be, nxt, ld, ws, ws_bytes,
-              stream);
+    rc = herm_product(n, cur, y, cur, nxt, ld, -0.5 * a * a * a, 1.5 * a, stream);
     if (rc) return rc;
-    if (i % 4 == 3 || i == nalpha - 1) {
-      rc = hermitize(nxt, ld, n, stream);
-      if (rc) return rc;
-    }
     double2* t = cur;
     cur = nxt;
     nxt = t;
   }
   *in_x2 = cur == static_cast<double2*>(x2) ? 1 : 0;
-  return QEPW_OK;
+  return herm_product(n, cur, cur, cur, y, ld, 1.0, 0.0, stream);
 }
 
 int set_diagonal(void* a, int64_t lda, int n, const double* d, hipStream_t stream) {

@@ -232,6 +232,112 @@ static SplitPlan split_plan(int op_a, int64_t m, int64_t n, int64_t k) {
   return {splits, kchunk, tile};
 }
 
+// Products of commuting Hermitian matrices (the Newton-Schulz steps of the
+// reduced eigensolve): C = alpha X Y + beta D with X, Y, D Hermitian
+// polynomials of one matrix, so that C is Hermitian.  Only the lower-triangle
+// 16x16 tiles are computed, one per workgroup, and each is written together
+// with its conjugate transpose (the diagonal made real): C is exactly
+// Hermitian, which the conjugated loads of Y below rely on.  Y's element
+// (k, j) is read as conj(Y[j, k]), a contiguous run like X's (i, k).  The four
+// waves of a workgroup split K, each fetching its next kHermUnroll MFMA
+// steps while the current ones run, and their partial tiles are summed in LDS
+// in wave order, so results are reproducible.
+constexpr int kHermUnroll = 4;
+
+__global__ __launch_bounds__(256) void k_herm_product(
+    int n, const double2* __restrict__ X, const double2* __restrict__ Y,
+    const double2* __restrict__ D, double2* __restrict__ C, int64_t ld, double alpha,
+    double beta) {
+  constexpr int U = kHermUnroll;
+  __shared__ double2 part[4][256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // tile (ti, tj), ti >= tj, of linear index b = ti (ti + 1) / 2 + tj
+  const int b = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= b) ++ti;
+  while (ti * (ti + 1) / 2 > b) --ti;
+  const int tj = b - ti * (ti + 1) / 2;
+  const int m0 = 16 * ti, n0 = 16 * tj;
+  // this wave's K steps (4 columns of K each): [s0, s1)
+  const int steps = (n + 3) >> 2;
+  const int per = (steps + 3) >> 2;
+  const int s0 = min(steps, wave * per);
+  const int s1 = min(steps, s0 + per);
+  const int r = lane & 15, kq = lane >> 4;
+  const int row = m0 + r, col = n0 + r;
+  const bool rok = row < n, cok = col < n;
+  const double2 zero = make_double2(0.0, 0.0);
+  v4d re = {0.0, 0.0, 0.0, 0.0}, im = {0.0, 0.0, 0.0, 0.0};
+  double2 a[U], w[U], an[U], wn[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int k = 4 * (s0 + u) + kq;
+    const bool ok = s0 + u < s1 && k < n;
+    a[u] = ok && rok ? X[row + (int64_t)k * ld] : zero;
+    w[u] = ok && cok ? Y[col + (int64_t)k * ld] : zero;
+  }
+  for (int s = s0; s < s1; s += U) {
+    const int sn = s + U;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = 4 * (sn + u) + kq;
+      const bool ok = sn + u < s1 && k < n;
+      an[u] = ok && rok ? X[row + (int64_t)k * ld] : zero;
+      wn[u] = ok && cok ? Y[col + (int64_t)k * ld] : zero;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      // (X Y)_ij += X_ik conj(Y_jk)
+      re = Mfma<double>::op(a[u].x, w[u].x, re);
+      re = Mfma<double>::op(a[u].y, w[u].y, re);
+      im = Mfma<double>::op(a[u].y, w[u].x, im);
+      im = Mfma<double>::op(-a[u].x, w[u].y, im);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      a[u] = an[u];
+      w[u] = wn[u];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    part[wave][Mfma<double>::row(lane, v) + 16 * r] = make_double2(re[v], im[v]);
+  __syncthreads();
+  const int er = tid & 15, ec = tid >> 4;
+  const int gi = m0 + er, gj = n0 + ec;
+  if (gi >= n || gj >= n || gi < gj) return;
+  double sr = 0.0, si = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    sr += part[q][tid].x;
+    si += part[q][tid].y;
+  }
+  double2 out = make_double2(alpha * sr, alpha * si);
+  if (beta != 0.0) {
+    const double2 d = D[gi + (int64_t)gj * ld];
+    out.x += beta * d.x;
+    out.y += beta * d.y;
+  }
+  if (gi == gj) {
+    out.y = 0.0;
+    C[gi + (int64_t)gj * ld] = out;
+  } else {
+    C[gi + (int64_t)gj * ld] = out;
+    C[gj + (int64_t)gi * ld] = make_double2(out.x, -out.y);
+  }
+}
+
+int herm_product(int n, const void* x, const void* y, const void* d, void* c, int64_t ld,
+                 double alpha, double beta, hipStream_t stream) {
+  if (n <= 0) return QEPW_OK;
+  if (ld < n) return QEPW_ERR_ARG;
+  const int nt = (n + 15) / 16;
+  hipLaunchKernelGGL(k_herm_product, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0,
+                     stream, n, static_cast<const double2*>(x), static_cast<const double2*>(y),
+                     static_cast<const double2*>(d), static_cast<double2*>(c), ld, alpha, beta);
+  return hipGetLastError() == hipSuccess ? QEPW_OK : QEPW_ERR_HIP;
+}
+
 size_t gemm_workspace_bytes(int precision, int op, int64_t m, int64_t n, int64_t k) {
   const SplitPlan sp = split_plan(op & QEPW_OP_C, m, n, k);
   if (sp.splits <= 1) return 0;

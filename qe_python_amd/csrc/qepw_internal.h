@@ -28,6 +28,8 @@ size_t gemm_workspace_bytes(int precision, int op_a, int64_t m, int64_t n, int64
 int gemm(int precision, int op_a, int64_t m, int64_t n, int64_t k, const double* alpha,
          const void* a, int64_t lda, const void* b, int64_t ldb, const double* beta,
          void* c, int64_t ldc, void* workspace, size_t ws_bytes, hipStream_t stream);
+int herm_product(int n, const void* x, const void* y, const void* d, void* c, int64_t ld,
+                 double alpha, double beta, hipStream_t stream);
 
 // qepw_vec.hip
 int column_squared_norms(int precision, const void* v, int64_t ldv, int64_t nrows, int nb,

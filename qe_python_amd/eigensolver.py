@@ -348,9 +348,9 @@ class DeviceRayleighRitz:
 
         def sign_steps(schedule):
             nonlocal X, X2
-            # the steps run back to back inside the library (X^3 is Hermitian
-            # up to rounding; it re-symmetrises every fourth step and after
-            # the last)
+            # the steps run back to back inside the library, as products of
+            # commuting Hermitian matrices whose lower triangles are computed
+            # and mirrored; Y is left holding the square of the result
             coeffs = (ctypes.c_double * len(schedule))(*schedule)
             nat.check(lib.qepw_sign_iteration(X.data_ptr(), X2.data_ptr(), Y.data_ptr(),
                                               self.maxsub, n, coeffs, len(schedule),
@@ -359,7 +359,6 @@ class DeviceRayleighRitz:
             if in_x2.value:
                 X, X2 = X2, X
             self.stats["ns_iterations"] += len(schedule)
-            self._gemm(nat.OP_N, n, n, n, 1.0, X, X, 0.0, Y)
             nat.check(lib.qepw_sign_check(X.data_ptr(), self.maxsub, Y.data_ptr(), self.maxsub, n,
                                           self.sig.data_ptr(), st), "qepw_sign_check")
 

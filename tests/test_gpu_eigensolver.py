@@ -75,13 +75,15 @@ def test_spectral_rayleigh_ritz_matches_host(nroots, blocks):
     assert used_spectral > 0, rr.stats
 
 
-def test_sign_iteration_entry_point():
-    """qepw_sign_iteration against the same scaled Newton-Schulz steps in numpy."""
+@pytest.mark.parametrize("n,ld", [(300, 320), (37, 40)])
+def test_sign_iteration_entry_point(n, ld):
+    """qepw_sign_iteration against the same scaled Newton-Schulz steps in numpy:
+    the iterate (exactly Hermitian) and its square, left in y.  300 and 37 are
+    not multiples of the 16 x 16 tiles nor of the K steps."""
     import ctypes
     from qe_python_amd import _native as nat
     from qe_python_amd import device as dv
     rng = np.random.default_rng(5)
-    n, ld = 300, 320  # large enough that the square GEMMs split K (workspace > 0)
     a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     a = a + a.conj().T
     x0 = a / np.abs(a).sum(axis=0).max()
@@ -103,12 +105,11 @@ def test_sign_iteration_entry_point():
     assert which.value == 1  # five steps: the result ping-ponged into x2
     got = bufs[which.value][:n, :n].cpu().numpy().T
     np.testing.assert_allclose(got, ref, atol=1e-12)
+    np.testing.assert_array_equal(got, got.conj().T)
+    sq = bufs[2][:n, :n].cpu().numpy().T
+    np.testing.assert_allclose(sq, got @ got, atol=1e-12)
     with pytest.raises(ValueError):
         nat.check(lib.qepw_sign_iteration(bufs[0].data_ptr(), bufs[0].data_ptr(),
                                           bufs[2].data_ptr(), ld, n, coeffs, 1,
                                           ctypes.byref(which), ws.data_ptr(), ws.numel(), st),
                   "sign")
-    with pytest.raises(RuntimeError):
-        nat.check(lib.qepw_sign_iteration(bufs[0].data_ptr(), bufs[1].data_ptr(),
-                                          bufs[2].data_ptr(), ld, n, coeffs, 1,
-                                          ctypes.byref(which), ws.data_ptr(), 0, st), "sign")
