@@ -225,8 +225,7 @@ class DeviceRayleighRitz:
         with _one_blas_thread():
             self._standardize(active)
             out = None
-            if self.theta is not None and active > k + 8 and (
-                    len(self.theta) >= k + 2 or self._can_keep_shift()):
+            if self.theta is not None and active > k + 8:
                 out = self._spectral(active)
             else:
                 self.stats["skipped"] += 1
@@ -317,6 +316,20 @@ class DeviceRayleighRitz:
             # shift) many more would, so a shift from the gaps is taken then
             sigma = self.sigma
             lo_v, hi_v = theta[-1], 2.0 * sigma - theta[-1]
+        elif len(theta) < k + 2:
+            # too few values known above the k-th for a gap: one mean spacing
+            # of the top values above the last.  The grown A has at least
+            # len(theta) >= k eigenvalues at or below theta[-1] (interlacing),
+            # so at least k lie below sigma; an eigenvalue close to sigma only
+            # slows the iteration, which the convergence check catches
+            if len(theta) < k:
+                return None
+            top = theta[max(0, len(theta) - 5):]
+            spacing = (top[-1] - top[0]) / max(1, len(top) - 1)
+            if not spacing > 0.0:
+                return None
+            sigma = theta[-1] + spacing
+            lo_v, hi_v = theta[-1], theta[-1] + 2.0 * spacing
         else:
             # a new shift in the widest gap among theta[k-1+margin .. hi-1]:
             # at least k + margin eigenvalues of the grown A lie below it
