@@ -242,7 +242,7 @@ static SplitPlan split_plan(int op_a, int64_t m, int64_t n, int64_t k) {
 // waves of a workgroup split K, each fetching its next kHermUnroll MFMA
 // steps while the current ones run, and their partial tiles are summed in LDS
 // in wave order, so results are reproducible.
-constexpr int kHermUnroll = 4;
+constexpr int kHermUnroll = 8;
 
 __global__ __launch_bounds__(256) void k_herm_product(
     int n, const double2* __restrict__ X, const double2* __restrict__ Y,
@@ -266,15 +266,24 @@ __global__ __launch_bounds__(256) void k_herm_product(
   const int r = lane & 15, kq = lane >> 4;
   const int row = m0 + r, col = n0 + r;
   const bool rok = row < n, cok = col < n;
-  const double2 zero = make_double2(0.0, 0.0);
   v4d re = {0.0, 0.0, 0.0, 0.0}, im = {0.0, 0.0, 0.0, 0.0};
-  double2 a[U], w[U], an[U], wn[U];
+  // operands are loaded unconditionally from an in-bounds element (the
+  // matrix corner when out of range) and zeroed by register selects, which
+  // keeps the loads on the global path and the operands in registers
+  const double2* xr = X + (rok ? row : 0);
+  const double2* yc = Y + (cok ? col : 0);
+  double ar[U], ai[U], wr[U], wi[U], nar[U], nai[U], nwr[U], nwi[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int k = 4 * (s0 + u) + kq;
     const bool ok = s0 + u < s1 && k < n;
-    a[u] = ok && rok ? X[row + (int64_t)k * ld] : zero;
-    w[u] = ok && cok ? Y[col + (int64_t)k * ld] : zero;
+    const int64_t off = ok ? (int64_t)k * ld : 0;
+    const double2 xa = xr[off], yw = yc[off];
+    const bool ua = ok && rok, uw = ok && cok;
+    ar[u] = ua ? xa.x : 0.0;
+    ai[u] = ua ? xa.y : 0.0;
+    wr[u] = uw ? yw.x : 0.0;
+    wi[u] = uw ? yw.y : 0.0;
   }
   for (int s = s0; s < s1; s += U) {
     const int sn = s + U;
@@ -282,21 +291,28 @@ __global__ __launch_bounds__(256) void k_herm_product(
     for (int u = 0; u < U; ++u) {
       const int k = 4 * (sn + u) + kq;
       const bool ok = sn + u < s1 && k < n;
-      an[u] = ok && rok ? X[row + (int64_t)k * ld] : zero;
-      wn[u] = ok && cok ? Y[col + (int64_t)k * ld] : zero;
+      const int64_t off = ok ? (int64_t)k * ld : 0;
+      const double2 xa = xr[off], yw = yc[off];
+      const bool ua = ok && rok, uw = ok && cok;
+      nar[u] = ua ? xa.x : 0.0;
+      nai[u] = ua ? xa.y : 0.0;
+      nwr[u] = uw ? yw.x : 0.0;
+      nwi[u] = uw ? yw.y : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       // (X Y)_ij += X_ik conj(Y_jk)
-      re = Mfma<double>::op(a[u].x, w[u].x, re);
-      re = Mfma<double>::op(a[u].y, w[u].y, re);
-      im = Mfma<double>::op(a[u].y, w[u].x, im);
-      im = Mfma<double>::op(-a[u].x, w[u].y, im);
+      re = Mfma<double>::op(ar[u], wr[u], re);
+      re = Mfma<double>::op(ai[u], wi[u], re);
+      im = Mfma<double>::op(ai[u], wr[u], im);
+      im = Mfma<double>::op(-ar[u], wi[u], im);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      a[u] = an[u];
-      w[u] = wn[u];
+      ar[u] = nar[u];
+      ai[u] = nai[u];
+      wr[u] = nwr[u];
+      wi[u] = nwi[u];
     }
   }
 #pragma unroll
