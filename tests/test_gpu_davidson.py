@@ -79,6 +79,52 @@ def test_matches_oracle_trajectory_closely(small):
     assert abs(res.iterations - ref.iterations) <= 3
 
 
+SCF_OPTIONS = [
+    # the SCF's keywords (pw/scf.py:3510-3537): occupied roots with the looser
+    # test for the empty ones (diagonalization.py:581-583), the
+    # residual-energy test, and smaller subspaces (more restarts)
+    dict(occupied_roots=4),
+    dict(residual_factor=None, residual_energy_scale=1.0),
+    dict(residual_factor=None, subspace_multiplier=2),
+    dict(occupied_roots=5, subspace_multiplier=3, max_iterations=60),
+]
+# with a smaller subspace the SCF passes nbnd starting vectors: 16 would leave
+# no room for the first corrections, which the reference rejects too
+
+
+@pytest.mark.parametrize("opts", SCF_OPTIONS)
+def test_scf_options_match_oracle(small, opts):
+    from types import SimpleNamespace
+
+    from qe_python_amd.diagonalization import davidson
+    from qe_python_amd.timing import TimingRegistry
+    s, ham, exact = small
+    nroots = 7
+    timers = TimingRegistry()
+    start = s.trial[:, :nroots] if "subspace_multiplier" in opts else s.trial
+    res = davidson(ham.apply, ham.diagonal, nroots, initial_vectors=start, tolerance=TOL_HA,
+                   operator_into=ham.apply_into, timers=timers, **opts)
+    bs = s.packed_term()
+    oh = OracleHamiltonian(s.basis.indices, s.basis.kinetic, s.shape, s.real_potential,
+                           (SimpleNamespace(beta=bs[0], coupling=bs[1], diagonal=bs[2]),),
+                           potential_average=float(np.real(s.potential_g.flat[0])))
+    ref = odav.davidson(oh.apply, oh.diagonal, nroots, initial_vectors=start,
+                        tolerance=TOL_HA, operator_into=oh.apply_into, **opts)
+    print(f"{opts}: device {res.iterations} iterations / {res.hamiltonian_applications} "
+          f"H applications, oracle {ref.iterations} / {ref.hamiltonian_applications}")
+    assert res.converged and ref.converged
+    occ = opts.get("occupied_roots", nroots)
+    # occupied roots to the solver tolerance; the others to max(5 tol, 5e-6)
+    loose = max(5.0 * TOL_HA, 5.0e-6)
+    err = np.abs(res.eigenvalues - exact[:nroots])
+    assert np.all(err[:occ] < 10 * TOL_HA), err
+    assert np.all(err[occ:] < 10 * loose), err
+    np.testing.assert_allclose(res.eigenvalues[:occ], ref.eigenvalues[:occ],
+                               atol=10 * TOL_HA, rtol=0)
+    assert abs(res.iterations - ref.iterations) <= 3
+    assert timers.entries, "timer labels were not recorded"
+
+
 def test_generic_operator_reference_pattern():
     """tests/unit/core/test_diagonalization.py:8-31 pattern on the device solver."""
     from qe_python_amd.diagonalization import davidson
