@@ -120,6 +120,16 @@ def check(rc: int, what: str) -> None:
     raise RuntimeError(msg)
 
 
+_PAIRS: dict = {}
+
+
 def complex_pair(z) -> ctypes.Array:
+    """(re, im) as a ctypes double[2]; the few coefficients in use are
+    cached (the arrays are only read by the library)."""
     z = complex(z)
-    return (c_double * 2)(z.real, z.imag)
+    pair = _PAIRS.get(z)
+    if pair is None:
+        pair = (c_double * 2)(z.real, z.imag)
+        if len(_PAIRS) < 1024:
+            _PAIRS[z] = pair
+    return pair

@@ -86,14 +86,18 @@ def require_device(device=None) -> torch.device:
 
 
 def ld_of(t: torch.Tensor) -> int:
-    if t.dim() != 2 or (t.shape[0] > 1 and t.shape[1] > 1 and t.stride(1) != 1):
+    if t.dim() != 2:
+        raise ValueError("device matrix must be a (ncols, ld) tensor with unit inner stride")
+    s0, s1 = t.shape
+    st0, st1 = t.stride()
+    if s0 > 1 and s1 > 1 and st1 != 1:
         raise ValueError("device matrix must be a (ncols, ld) tensor with unit inner stride")
     if t.is_conj() or t.is_neg():
         # a lazy conjugate / negative view: its memory holds the plain values
         raise ValueError("device matrix is a lazy conj/neg view; resolve it first")
-    if t.shape[0] > 1:
-        return max(1, int(t.stride(0)))
-    return max(1, int(t.shape[1]), int(t.stride(0)))
+    if s0 > 1:
+        return max(1, st0)
+    return max(1, s1, st0)
 
 
 class Workspace:
@@ -112,6 +116,10 @@ class Workspace:
         return self.buffer
 
 
+# workspace bytes of a GEMM shape (a pure function of the shape)
+_GEMM_WORKSPACE: dict = {}
+
+
 def gemm(op_a: int, m: int, n: int, k: int, alpha, a: torch.Tensor, b: torch.Tensor,
          beta, c: torch.Tensor, ws: Workspace) -> None:
     """c = alpha op(a) b + beta c on the device (column-major views).
@@ -121,20 +129,28 @@ def gemm(op_a: int, m: int, n: int, k: int, alpha, a: torch.Tensor, b: torch.Ten
     tensor), so that no size slip reaches the device as an access past an
     allocation."""
     lib = nat.load()
-    prec = precision_code(c.dtype)
-    if a.dtype != c.dtype or b.dtype != c.dtype:
+    dt = c.dtype
+    if a.dtype != dt or b.dtype != dt:
         raise ValueError("gemm operands must share one complex dtype")
-    if min(m, n, k) > 0:
+    prec = precision_code(dt)
+    lda, ldb, ldc = ld_of(a), ld_of(b), ld_of(c)
+    if m > 0 and n > 0 and k > 0:
         a_shape = (m, k) if op_a & nat.OP_C else (k, m)
         b_shape = (k, n) if op_a & nat.OP_BC else (n, k)
         for name, t, (cols, rows) in (("a", a, a_shape), ("b", b, b_shape), ("c", c, (n, m))):
-            if t.dim() != 2 or t.shape[0] < cols or t.shape[1] < rows:
+            if t.shape[0] < cols or t.shape[1] < rows:
                 raise ValueError(f"gemm operand {name} of shape {tuple(t.shape)} is smaller "
                                  f"than the {rows} x {cols} matrix the sizes name")
-    need = lib.qepw_gemm_workspace(prec, op_a, m, n, k)
+    key = (prec, op_a, m, n, k)
+    need = _GEMM_WORKSPACE.get(key)
+    if need is None:
+        need = int(lib.qepw_gemm_workspace(prec, op_a, m, n, k))
+        if len(_GEMM_WORKSPACE) > 8192:
+            _GEMM_WORKSPACE.clear()
+        _GEMM_WORKSPACE[key] = need
     buf = ws.get(need)
-    rc = lib.qepw_gemm(prec, op_a, m, n, k, nat.complex_pair(alpha), a.data_ptr(), ld_of(a),
-                       b.data_ptr(), ld_of(b), nat.complex_pair(beta), c.data_ptr(), ld_of(c),
+    rc = lib.qepw_gemm(prec, op_a, m, n, k, nat.complex_pair(alpha), a.data_ptr(), lda,
+                       b.data_ptr(), ldb, nat.complex_pair(beta), c.data_ptr(), ldc,
                        buf.data_ptr(), buf.numel(), stream_handle(c.device))
     nat.check(rc, "qepw_gemm")
 

@@ -63,6 +63,15 @@ def _one_blas_thread():
     return _THREADPOOLS.limit(limits=1, user_api="blas")
 
 
+def _lower_inverse(lower: np.ndarray) -> np.ndarray:
+    """Inverse of a nonsingular lower-triangular complex matrix (LAPACK
+    ztrtri; its strictly upper part is zero)."""
+    inv, info = sla.lapack.ztrtri(np.asarray(lower, dtype=np.complex128), lower=1)
+    if info != 0:
+        raise np.linalg.LinAlgError(f"ztrtri failed (info {info})")
+    return np.tril(inv)
+
+
 def _scaled_ns_alphas(lower: float, tol: float = 1e-15, limit: int = 60) -> list[float]:
     """Chen-Chow scaling factors for the sign iteration from a lower bound
     ``lower`` of |eigenvalues| of X0 (all |x| <= 1)."""
@@ -104,7 +113,7 @@ class DeviceRayleighRitz:
         self.stats = {"spectral": 0, "dense": 0, "host": 0, "ns_iterations": 0,
                       "max_check_error": 0.0, "skipped": 0, "ns_unconverged": 0,
                       "too_few_below": 0, "gram_not_pd": 0}
-        self._rng = np.random.default_rng(1234)
+        self._gauss = None           # Gaussian sample columns (device, drawn once)
 
     # -- views -------------------------------------------------------------
     def block(self, mat: torch.Tensor, r0: int, r1: int, c0: int, c1: int) -> torch.Tensor:
@@ -189,7 +198,7 @@ class DeviceRayleighRitz:
             l22 = np.linalg.cholesky(m)
         except np.linalg.LinAlgError as exc:
             raise np.linalg.LinAlgError(f"zhegvd failed: overlap not positive definite ({exc})")
-        w22 = sla.solve_triangular(l22, np.eye(nnew), lower=True)
+        w22 = _lower_inverse(l22)
         w22_dev = dv.upload(w22.T, self.device)
         self.block(self.W, first, active, first, active).copy_(w22_dev)
         # the strictly upper part of the new columns stays zero
@@ -378,15 +387,22 @@ class DeviceRayleighRitz:
         def sampled_range(count):
             """Q0 = (I - X) Omega / 2 for Omega = [previous eigenvectors,
             Gaussian columns] (count columns) and its Gram matrix."""
-            omega = torch.zeros((count, self.maxsub), dtype=C128, device=self.device)
+            # rows [:count] and columns [:n] are all written below
+            omega = torch.empty((count, self.maxsub), dtype=C128, device=self.device)
             start = 0
             if self.u_prev is not None:
                 start = min(k, count)
                 omega[:start].copy_(self.u_prev[:start])
             if start < count:
-                r = self._rng.standard_normal((count - start, n)) + 1j * self._rng.standard_normal(
-                    (count - start, n))
-                omega[start:count, :n] = dv.upload(r, self.device)
+                if self._gauss is None:
+                    # one seeded draw per reduced problem, on the device: the
+                    # sample only has to be generic, and reusing it keeps the
+                    # host out of the loop (and runs reproducible)
+                    gen = torch.Generator(device=self.device)
+                    gen.manual_seed(1234)
+                    self._gauss = torch.randn((self.maxsub, self.maxsub), dtype=C128,
+                                              device=self.device, generator=gen)
+                omega[start:count, :n].copy_(self._gauss[start:count, :n])
             Om = omega[:, :n]
             Q0 = Om.clone()
             self._gemm(nat.OP_N, n, count, n, -0.5, X, Om, 0.5, Q0)
@@ -433,7 +449,7 @@ class DeviceRayleighRitz:
         except np.linalg.LinAlgError:
             self.stats["gram_not_pd"] += 1
             return None
-        t = sla.solve_triangular(lg, np.eye(mp), lower=True).conj().T
+        t = _lower_inverse(lg).conj().T
         t_dev = dv.upload(t.T, self.device)
         Q = torch.empty_like(Q0)
         self._gemm(nat.OP_N, n, mp, mp, 1.0, Q0, t_dev, 0.0, Q)
