@@ -178,19 +178,45 @@ __global__ __launch_bounds__(kGemmThreads) void k_cgemm(
       }
 }
 
+// Sums the split-K partial tiles: a block owns 64 consecutive outputs and
+// each of its four waves adds every fourth split (eight loads in flight per
+// lane); the four partial sums are combined in LDS in wave order, so the
+// summation order is fixed and results are reproducible.
 template <typename T>
 __global__ __launch_bounds__(256) void k_reduce_splits(
     int64_t M, int64_t N, int splits, const cplx<T>* __restrict__ partial,
     cplx<T>* __restrict__ C, int64_t ldc, T alpha_r, T alpha_i, T beta_r, T beta_i) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= M * N) return;
-  const int64_t col = idx / M, row = idx - col * M;
+  __shared__ T sr_part[4][64], si_part[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t total = M * N;
+  const int64_t idx = (int64_t)blockIdx.x * 64 + lane;
   T sr = T(0), si = T(0);
-  for (int z = 0; z < splits; ++z) {
-    const cplx<T> p = partial[(int64_t)z * M * N + idx];
-    sr += p.x;
-    si += p.y;
+  if (idx < total) {
+    const cplx<T>* p = partial + idx;
+    int z = wave;
+    for (; z + 28 < splits; z += 32) {
+      cplx<T> v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = p[(int64_t)(z + 4 * u) * total];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        sr += v[u].x;
+        si += v[u].y;
+      }
+    }
+    for (; z < splits; z += 4) {
+      const cplx<T> v = p[(int64_t)z * total];
+      sr += v.x;
+      si += v.y;
+    }
   }
+  sr_part[wave][lane] = sr;
+  si_part[wave][lane] = si;
+  __syncthreads();
+  if (wave != 0 || idx >= total) return;
+  sr = sr_part[0][lane] + sr_part[1][lane] + sr_part[2][lane] + sr_part[3][lane];
+  si = si_part[0][lane] + si_part[1][lane] + si_part[2][lane] + si_part[3][lane];
+  const int64_t col = idx / M, row = idx - col * M;
   cplx<T> out;
   out.x = alpha_r * sr - alpha_i * si;
   out.y = alpha_r * si + alpha_i * sr;
@@ -216,11 +242,15 @@ static SplitPlan split_plan(int op_a, int64_t m, int64_t n, int64_t k) {
   const int64_t tiles = ((m + tile - 1) / tile) * ((n + tile - 1) / tile);
   int64_t splits = 1;
   // split K when the output tiles alone cannot fill the 256 CUs: the tall
-  // A^H B products over npw and the small square products of the reduced
-  // eigensolve
-  if ((op_a == QEPW_OP_C && k > 512) || (tiles < 256 && k >= 256)) {
-    const int64_t want = (1024 + tiles - 1) / tiles;
-    const int64_t most = k / 128;
+  // A^H B products over npw (about two workgroups per CU and at least 256 of
+  // K per split: shorter splits spend more on writing and summing their
+  // partial tiles than on the product) and the small square products of the
+  // reduced eigensolve (small partial tiles: four workgroups per CU, at
+  // least 128 of K per split)
+  const bool tall = op_a == QEPW_OP_C && k > 2048;
+  if (tall || (op_a == QEPW_OP_C && k > 512) || (tiles < 256 && k >= 256)) {
+    const int64_t want = ((tall ? 512 : 1024) + tiles - 1) / tiles;
+    const int64_t most = k / (tall ? 256 : 128);
     splits = want < most ? want : most;
     if (splits < 1) splits = 1;
   }
@@ -393,7 +423,7 @@ static int gemm_t(int op, int64_t m, int64_t n, int64_t k, const double* alpha,
 #undef QEPW_GEMM_TILE
   if (sp.splits > 1) {
     const int64_t total = m * n;
-    hipLaunchKernelGGL(k_reduce_splits<T>, dim3((unsigned)((total + 255) / 256)), dim3(256),
+    hipLaunchKernelGGL(k_reduce_splits<T>, dim3((unsigned)((total + 63) / 64)), dim3(256),
                        0, stream, m, n, (int)sp.splits, part, static_cast<cplx<T>*>(c), ldc,
                        ar, ai, br, bi);
   }

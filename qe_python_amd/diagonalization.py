@@ -47,7 +47,7 @@ import torch
 
 from . import _native as nat
 from . import device as dv
-from .eigensolver import DeviceRayleighRitz, _one_blas_thread
+from .eigensolver import DeviceRayleighRitz, _lower_inverse, _one_blas_thread
 
 # default reduced eigensolve of ``davidson``: "device" (GPU spectral
 # division) or "host" (LAPACK zhegvd, the reference's routine)
@@ -514,7 +514,7 @@ def _cholesky_orthonormalize(block: torch.Tensor, npw: int, ws, condition: float
         d = np.abs(np.diag(chol))
         if d.min() <= 0.0 or (d.max() / d.min()) ** 2 > condition:
             return None
-        inv = sla.solve_triangular(chol, np.eye(n), lower=True).conj().T   # L^-H
+        inv = _lower_inverse(chol).conj().T   # L^-H
     t_dev = dv.upload(inv.T, device).to(block.dtype)
     out = torch.empty_like(block)
     dv.gemm(nat.OP_N, npw, n, n, 1.0, block, t_dev, 0.0, out, ws)
