@@ -109,6 +109,7 @@ class DeviceRayleighRitz:
         self.drift = None            # max change of the lowest k Ritz values between solves
         self._last_k = None
         self._bound_prev = None      # ||A - sigma I||_1 of the previous solve (host)
+        self._slow_shift = False     # the previous sign iteration needed extra steps
         self.u_prev = None           # previous lowest eigenvectors of A (device, (k, maxsub))
         self.stats = {"spectral": 0, "dense": 0, "host": 0, "ns_iterations": 0,
                       "max_check_error": 0.0, "skipped": 0, "ns_unconverged": 0,
@@ -294,7 +295,10 @@ class DeviceRayleighRitz:
 
     def _can_keep_shift(self) -> bool:
         theta, k = self.theta, self.nroots
-        return (self.sigma is not None and theta is not None
+        # a shift whose last sign iteration needed the extra steps has an
+        # eigenvalue close to it (Ritz values above it drift down towards it
+        # as the basis grows): a new one is taken in a gap instead
+        return (self.sigma is not None and theta is not None and not self._slow_shift
                 and k <= len(theta) <= k + self.extra and theta[-1] < self.sigma
                 and self.drift is not None and self.drift < 0.1 * (self.sigma - theta[-1]))
 
@@ -421,6 +425,7 @@ class DeviceRayleighRitz:
         res2, trace, self._bound_prev = (float(v) for v in packed[:3])
         g = packed[3:].view(np.complex128).reshape(count, count).T
         attempt = 0
+        self._slow_shift = res2 >= 1e-20 * n
         while res2 >= 1e-20 * n and attempt < 2:
             sign_steps([1.0] * 12)
             res2, trace = (float(v) for v in self.sig[:2].cpu().numpy())
