@@ -30,9 +30,15 @@ FLAGS = [
 ]
 
 
+# per-file flags: the FFT kernels feed literal zeros into their register DFTs
+# for the grid slots outside the sphere, and x + 0 folds to x only when the
+# sign of zero may be ignored
+EXTRA_FLAGS = {"qepw_fft.hip": ["-fno-signed-zeros"]}
+
+
 def _compile(src: Path) -> Path:
     obj = OBJ / (src.stem + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *FLAGS, *EXTRA_FLAGS.get(src.name, []), "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stderr[-6000:]}")

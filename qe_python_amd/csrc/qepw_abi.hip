@@ -201,6 +201,11 @@ int qepw_plan_create(const int32_t* miller, int64_t npw, int n1, int n2, int n3,
   for (int a = 0; a < 3; ++a) p.line[a] = lp[a];
   p.sphere.nst = nst;
   p.sphere.nxcols = (int)sphere.xcols.size();
+  if (n1 == n2) {
+    std::vector<int> stick_y(nst);
+    for (int s = 0; s < nst; ++s) stick_y[s] = sphere.plane[s] % n2;
+    p.sphere.slot_mask = qepw::team_slot_mask(n1, sphere.xcols, stick_y);
+  }
   p.sphere.npoints = npw;
   p.sphere.group = qepw::stick_group_for(n3, nst);
   p.sphere.ngroups = (nst + p.sphere.group - 1) / p.sphere.group;

@@ -146,12 +146,19 @@ __global__ __launch_bounds__(kPlaneThreads) void k_planes(
 
 // Fast xy-plane kernel for square planes (n1 = n2 = N): the LDS holds only
 // the nxc active x columns of the plane, Q[c * NP + y], plus one all-zero
-// column; every 1-D FFT is a register "team" FFT (qepw_regfft.h).  The row
-// pass chains the backward x-FFT (DIT), the V(r) multiply and the forward
-// x-FFT (DIF) in registers -- the DIF consumes the DIT's output order and
-// returns the DIT's input order -- so a row costs two team exchanges; the
-// column passes alternate DIT and DIF so that a team's consecutive lines need
-// one wave sync each.  Four workgroup barriers per plane.
+// column; every 1-D FFT is a register "team" FFT (qepw_regfft.h).
+//
+// The backward transforms are decimation-in-frequency team FFTs, whose input
+// slot s of lane t is position t + M s: the sphere's x and y extents leave
+// whole slots empty for every lane (with the usual grids only s = 0, 1, L-2,
+// L-1 hold sticks), so with KEEP naming the occupied slots the empty inputs
+// are literal zeros that the compiler folds away (this file is built with
+// -fno-signed-zeros for that), and the loads of them disappear.  The forward
+// transforms are decimation-in-time team FFTs, whose output slot s of lane t
+// is again position t + M s: only the kept slots are computed and stored.
+// The row pass chains backward x-FFT (DIF), V(r) multiply and forward x-FFT
+// (DIT) in registers -- the DIT consumes the DIF's output order -- at two
+// team exchanges per row.  Four workgroup barriers per plane.
 constexpr int kTeamThreads = 256;
 
 template <int N> struct TeamGeom {
@@ -159,49 +166,19 @@ template <int N> struct TeamGeom {
   static constexpr int M = N / L;
   static constexpr int per_wave = 64 / M;
   static constexpr int teams = (kTeamThreads / 64) * per_wave;
+  // input slots kept by the pruned kernel: the first two and the last two
+  static constexpr unsigned all = (1u << L) - 1u;
+  static constexpr unsigned pruned = L >= 6 ? (0x3u | (0x3u << (L - 2))) : all;
 };
 
-// One y-column of the plane, in place: DIT for even rounds, DIF for odd ones.
-template <typename T, int N, int SIGN>
-__device__ __forceinline__ void team_column(int t, bool dif, cplx<T>* col, cplx<T>* S,
-                                            const TeamTwiddles<T, N>& tw) {
-  constexpr int L = team_L(N);
-  constexpr int M = N / L;
-  cplx<T> a[M], b[L];
-  if (!dif) {
-    if (t < L) {
-#pragma unroll
-      for (int s = 0; s < M; ++s) a[s] = col[t + L * s];
-    }
-    dit_front<T, N, SIGN>(t, S, tw, a);
-    wave_lds_sync();
-    dit_back<T, N, SIGN>(t, S, b);
-    if (t < M) {
-#pragma unroll
-      for (int k = 0; k < L; ++k) col[t + M * k] = b[k];
-    }
-  } else {
-    if (t < M) {
-#pragma unroll
-      for (int s = 0; s < L; ++s) b[s] = col[t + M * s];
-    }
-    dif_front<T, N, SIGN>(t, S, tw, b);
-    wave_lds_sync();
-    dif_back<T, N, SIGN>(t, S, a);
-    if (t < L) {
-#pragma unroll
-      for (int k = 0; k < M; ++k) col[t + L * k] = a[k];
-    }
-  }
-}
-
-template <typename T, int N>
+template <typename T, int N, bool PRUNE>
 __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
     cplx<T>* __restrict__ work, int64_t band_stride, int nst,
     const int* __restrict__ stick_q, const int* __restrict__ colmap, int nxc,
     const T* __restrict__ V, const cplx<T>* __restrict__ tw) {
   using G = TeamGeom<N>;
   constexpr int L = G::L, M = G::M;
+  constexpr unsigned KEEP = PRUNE ? G::pruned : G::all;
   extern __shared__ unsigned char smem_raw[];
   // Q[c * NP + y]: active column c, padded stride NP = N + 1 so that the
   // columns (and the teams' scratch lines) start on different LDS banks.
@@ -224,14 +201,14 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
   cplx<T>* w = work + b * band_stride + (int64_t)z * nst;
 
   // per-lane tables, the same for every row of the plane: twiddles, and the
-  // Q offsets of this lane's row inputs x = t + L s (lanes t < L), which are
-  // also the row outputs it produces
+  // Q offsets of this lane's row slots x = t + M s (lanes t < M, kept s),
+  // which are both the backward inputs and the forward outputs
   TeamTwiddles<T, N> twr;
   twr.load(t, tw);
-  int cq[M];
+  int cq[L];
 #pragma unroll
-  for (int s = 0; s < M; ++s) {
-    const int c = t < L ? colmap[t + L * s] : -1;
+  for (int s = 0; s < L; ++s) {
+    const int c = (t < M && ((KEEP >> s) & 1u)) ? colmap[t + M * s] : -1;
     cq[s] = (c < 0 ? nxc : c) * NP;
   }
 
@@ -245,55 +222,97 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
 
   // backward y-FFT of every active column (in place in Q)
   if (in_team) {
-    bool dif = false;
-    for (int c = team; c < nxc; c += G::teams, dif = !dif)
-      team_column<T, N, +1>(t, dif, Q + c * NP, S, twr);
-  }
-  __syncthreads();
-
-  // rows: backward x-FFT (input only at active columns), V, forward x-FFT
-  // (output only at active columns)
-  if (in_team) {
-    const T* vz = V + (int64_t)z * N * N;
-    for (int y = team; y < N; y += G::teams) {
-      // V(r) at this lane's backward outputs x = t + M k, fetched first
-      T vv[L];
-#pragma unroll
-      for (int k = 0; k < L; ++k) vv[k] = t < M ? vz[y * N + t + M * k] : T(0);
-      cplx<T> a[M], bb[L];
-      if (t < L) {
-#pragma unroll
-        for (int s = 0; s < M; ++s) a[s] = Q[cq[s] + y];
-      }
-      dit_front<T, N, +1>(t, S, twr, a);
-      wave_lds_sync();
-      dit_back<T, N, +1>(t, S, bb);
+    for (int c = team; c < nxc; c += G::teams) {
+      // a team's consecutive DIFs share its scratch line
+      if (c != team) wave_lds_sync();
+      cplx<T>* col = Q + c * NP;
+      cplx<T> bb[L], a[M];
       if (t < M) {
 #pragma unroll
-        for (int k = 0; k < L; ++k) bb[k] = cscale(bb[k], vv[k]);
+        for (int s = 0; s < L; ++s) bb[s] = ((KEEP >> s) & 1u) ? col[t + M * s] : zero;
       }
-      dif_front<T, N, -1>(t, S, twr, bb);
+      dif_front<T, N, +1>(t, S, twr, bb);
       wave_lds_sync();
-      dif_back<T, N, -1>(t, S, a);
+      dif_back<T, N, +1>(t, S, a);
       if (t < L) {
 #pragma unroll
-        for (int s = 0; s < M; ++s) Q[cq[s] + y] = a[s];
+        for (int k = 0; k < M; ++k) col[t + L * k] = a[k];
       }
     }
   }
   __syncthreads();
 
-  // forward y-FFT of every active column
+  // rows: backward x-FFT (inputs only at active columns), V, forward x-FFT
+  // (outputs only at active columns)
   if (in_team) {
-    bool dif = false;
-    for (int c = team; c < nxc; c += G::teams, dif = !dif)
-      team_column<T, N, -1>(t, dif, Q + c * NP, S, twr);
+    const T* vz = V + (int64_t)z * N * N;
+    for (int y = team; y < N; y += G::teams) {
+      // V(r) at this lane's backward outputs x = t + L k, fetched first
+      T vv[M];
+#pragma unroll
+      for (int k = 0; k < M; ++k) vv[k] = t < L ? vz[y * N + t + L * k] : T(0);
+      cplx<T> bb[L], a[M];
+      if (t < M) {
+#pragma unroll
+        for (int s = 0; s < L; ++s) bb[s] = ((KEEP >> s) & 1u) ? Q[cq[s] + y] : zero;
+      }
+      dif_front<T, N, +1>(t, S, twr, bb);
+      wave_lds_sync();
+      dif_back<T, N, +1>(t, S, a);
+      if (t < L) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) a[k] = cscale(a[k], vv[k]);
+      }
+      dit_front<T, N, -1>(t, S, twr, a);
+      wave_lds_sync();
+      dit_back<T, N, -1>(t, S, bb);
+      if (t < M) {
+#pragma unroll
+        for (int s = 0; s < L; ++s)
+          if ((KEEP >> s) & 1u) Q[cq[s] + y] = bb[s];
+      }
+    }
+  }
+  __syncthreads();
+
+  // forward y-FFT of every active column; only the kept y slots, which hold
+  // every stick, are computed
+  if (in_team) {
+    for (int c = team; c < nxc; c += G::teams) {
+      if (c != team) wave_lds_sync();
+      cplx<T>* col = Q + c * NP;
+      cplx<T> a[M], bb[L];
+      if (t < L) {
+#pragma unroll
+        for (int s = 0; s < M; ++s) a[s] = col[t + L * s];
+      }
+      dit_front<T, N, -1>(t, S, twr, a);
+      wave_lds_sync();
+      dit_back<T, N, -1>(t, S, bb);
+      if (t < M) {
+#pragma unroll
+        for (int k = 0; k < L; ++k)
+          if ((KEEP >> k) & 1u) col[t + M * k] = bb[k];
+      }
+    }
   }
   __syncthreads();
   for (int s = tid; s < nst; s += kTeamThreads) {
     const int q = stick_q[s];
     w[s] = Q[q + q / N];
   }
+}
+
+// Slots of the team FFT of length n that hold any active x column or stick y
+// (bit s: positions t + M s, t < M); 0 when n has no team FFT.
+unsigned team_slot_mask(int n, const std::vector<int>& xs, const std::vector<int>& ys) {
+  const int l = team_L(n);
+  if (l <= 0) return 0u;
+  const int m = n / l;
+  unsigned mask = 0u;
+  for (int x : xs) mask |= 1u << (x / m);
+  for (int y : ys) mask |= 1u << (y / m);
+  return mask;
 }
 
 // z-stage kernels with register team FFTs for the lengths in QEPW_TEAM_SIZES:
@@ -413,16 +432,24 @@ static bool launch_planes_team(const Plan& p, const StickTables& st, cplx<T>* wo
   const size_t lds = team_lds_bytes<T>(p.n1, st.nxcols);
   if (lds == 0 || lds > (size_t)kMaxLdsBytes) return false;
   const dim3 grid(p.n3, nb);
-#define QEPW_LAUNCH_TEAM(NN)                                                              \
-  if (p.n1 == NN) {                                                                       \
-    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_planes_team<T, NN>),            \
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);            \
-    hipLaunchKernelGGL((k_planes_team<T, NN>), grid, dim3(kTeamThreads), lds, stream,     \
-                       work, bstride, st.nst, st.d_stickq, st.d_colmap, st.nxcols, V, tw); \
-    return true;                                                                          \
+#define QEPW_LAUNCH_TEAM_K(NN, PR)                                                           \
+  do {                                                                                       \
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_planes_team<T, NN, PR>),           \
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);               \
+    hipLaunchKernelGGL((k_planes_team<T, NN, PR>), grid, dim3(kTeamThreads), lds, stream,    \
+                       work, bstride, st.nst, st.d_stickq, st.d_colmap, st.nxcols, V, tw);  \
+  } while (0)
+#define QEPW_LAUNCH_TEAM(NN)                                                                 \
+  if (p.n1 == NN) {                                                                          \
+    if ((st.slot_mask & ~TeamGeom<NN>::pruned) == 0u)                                        \
+      QEPW_LAUNCH_TEAM_K(NN, true);                                                          \
+    else                                                                                     \
+      QEPW_LAUNCH_TEAM_K(NN, false);                                                         \
+    return true;                                                                             \
   }
   QEPW_TEAM_SIZES(QEPW_LAUNCH_TEAM)
 #undef QEPW_LAUNCH_TEAM
+#undef QEPW_LAUNCH_TEAM_K
   return false;
 }
 

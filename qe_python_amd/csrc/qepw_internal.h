@@ -19,6 +19,7 @@ size_t density_workspace_bytes(const Plan& p, int precision, int nb);
 int accumulate_density(const Plan& p, int precision, const void* vectors, int64_t ldv, int nb,
                        const double* weights, double scale, double* out, void* workspace,
                        size_t ws_bytes, hipStream_t stream);
+unsigned team_slot_mask(int n, const std::vector<int>& xs, const std::vector<int>& ys);
 size_t inverse_real_bytes(const Plan& p);
 int inverse_real(const Plan& p, const void* grid_g, void* out_real, void* workspace,
                  size_t ws_bytes, hipStream_t stream);

@@ -30,6 +30,7 @@ struct StickTables {
   int* d_xcols = nullptr;  // nxcols: active x values
   int* d_stickq = nullptr;  // nst: c * n2 + y, c = rank of the stick's x in xcols
   int* d_colmap = nullptr;  // n1: rank of x among active columns, or -1
+  unsigned slot_mask = 0;   // team-FFT input slots holding an active x or stick y
 };
 
 struct Plan {
