@@ -353,6 +353,18 @@ class DeviceRayleighRitz:
             j = int(np.argmax(gaps))
             lo_v, hi_v = theta[lo_i + j], theta[lo_i + j + 1]
             sigma = 0.5 * (lo_v + hi_v)
+            spacing = (theta[hi - 1] - theta[k - 1]) / max(1, hi - k)
+            if self.drift is not None and self.drift > 3.0 * spacing:
+                # the Ritz values still move by several spacings of the
+                # window: the old gaps say little about the grown A, whose new
+                # eigenvalues crowd in above theta[k-1] (up to one per new
+                # basis vector), so the shift goes just above theta[k-1] --
+                # still at least k eigenvalues below it, and far fewer for
+                # the host eigensolve; a close eigenvalue only costs sign
+                # iteration steps
+                gap = theta[k] - theta[k - 1]
+                lo_v, hi_v = theta[k - 1], theta[k - 1] + 0.1 * gap
+                sigma = 0.5 * (lo_v + hi_v)
         A = self.block(self.A, 0, n, 0, n)
         X, X2, Y = (self.block(m, 0, n, 0, n) for m in (self.X, self.X2, self.Y))
         nat.check(lib.qepw_shift_scale(A.data_ptr(), self.maxsub, n, sigma, X.data_ptr(),
