@@ -70,6 +70,13 @@ int qepw_plan_destroy(qepw_plan* plan);
 int qepw_plan_query(const qepw_plan* plan, int64_t* info);
 /* perm[d] (host int32, npw) = reference row of device row d. */
 int qepw_plan_permutation(const qepw_plan* plan, int32_t* perm);
+/* QEPW_OK when qepw_apply_local can run this plan in ``precision``, else
+ * QEPW_ERR_UNSUPPORTED with a message naming the grid (the FFT kernels hold
+ * an xy plane's active columns, or a whole plane for non-square planes, and
+ * a group of z sticks in one workgroup's LDS).  The reference's FFTW plans
+ * (NativeFFTWPlan, qepy_pw/_native_fft.pyx:1973-2086) take any grid; callers
+ * check here once per plan instead of at the first h_psi. */
+int qepw_plan_supported(const qepw_plan* plan, int precision);
 /* Device copy of the same permutation (int32, npw). */
 const int32_t* qepw_plan_device_permutation(const qepw_plan* plan);
 

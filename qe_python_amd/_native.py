@@ -35,6 +35,7 @@ SIGNATURES = [
     ("qepw_plan_destroy", c_int, [c_void_p]),
     ("qepw_plan_query", c_int, [c_void_p, POINTER(c_int64)]),
     ("qepw_plan_permutation", c_int, [c_void_p, c_void_p]),
+    ("qepw_plan_supported", c_int, [c_void_p, c_int]),
     ("qepw_plan_device_permutation", c_void_p, [c_void_p]),
     ("qepw_apply_local_workspace", c_size_t, [c_void_p, c_int, c_int]),
     ("qepw_apply_local", c_int, [c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p, c_void_p,

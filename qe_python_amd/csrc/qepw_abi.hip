@@ -286,6 +286,28 @@ size_t qepw_apply_local_workspace(const qepw_plan* plan, int precision, int nb) 
   return qepw::apply_local_band_bytes(plan->p, precision) * (size_t)nb;
 }
 
+namespace {
+// the message of a local-potential plan whose planes the kernels cannot hold
+int unsupported_grid(const qepw::Plan& p, const char* where) {
+  g_error = std::string(where) + ": the " + std::to_string(p.n1) + " x " + std::to_string(p.n2) +
+            " x " + std::to_string(p.n3) +
+            " grid is outside the FFT kernels' range (an xy plane's active columns and "
+            "exchange lines, or a z stick group, must fit one workgroup's 160 KB of LDS)";
+  return QEPW_ERR_UNSUPPORTED;
+}
+}  // namespace
+
+int qepw_plan_supported(const qepw_plan* plan, int precision) {
+  if (!plan) return fail(QEPW_ERR_ARG, "qepw_plan_supported: NULL plan");
+  if (precision != QEPW_C128 && precision != QEPW_C64)
+    return fail(QEPW_ERR_ARG, "qepw_plan_supported: unknown precision");
+  // no bands: only the capability checks of qepw_apply_local run
+  const int rc = qepw::apply_local(plan->p, precision, nullptr, 0, 0, nullptr, nullptr, nullptr,
+                                   0, nullptr, 0, nullptr);
+  if (rc == QEPW_ERR_UNSUPPORTED) return unsupported_grid(plan->p, "qepw_plan_supported");
+  return finish(rc, "qepw_plan_supported");
+}
+
 int qepw_apply_local(const qepw_plan* plan, int precision, const void* vectors, int64_t ldv,
                      int nb, const void* potential, const void* kinetic, void* result,
                      int64_t ldr, void* workspace, size_t ws_bytes, void* stream) {
@@ -297,10 +319,11 @@ int qepw_apply_local(const qepw_plan* plan, int precision, const void* vectors, 
   if (ldv < plan->p.npw || ldr < plan->p.npw)
     return fail(QEPW_ERR_ARG, "qepw_apply_local: leading dimension smaller than npw");
   if (vectors == result) return fail(QEPW_ERR_ARG, "qepw_apply_local: result aliases input");
-  return finish(qepw::apply_local(plan->p, precision, vectors, ldv, nb, potential, kinetic,
-                                  result, ldr, workspace, ws_bytes,
-                                  static_cast<hipStream_t>(stream)),
-                "qepw_apply_local");
+  const int rc = qepw::apply_local(plan->p, precision, vectors, ldv, nb, potential, kinetic,
+                                   result, ldr, workspace, ws_bytes,
+                                   static_cast<hipStream_t>(stream));
+  if (rc == QEPW_ERR_UNSUPPORTED) return unsupported_grid(plan->p, "qepw_apply_local");
+  return finish(rc, "qepw_apply_local");
 }
 
 size_t qepw_inverse_real_workspace(const qepw_plan* plan) {

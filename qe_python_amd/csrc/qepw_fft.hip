@@ -409,8 +409,8 @@ __global__ __launch_bounds__(kTeamThreads) void k_sticks_forward_team(
 
 // Square plane sizes with an instantiated fast kernel.
 #define QEPW_TEAM_SIZES(X) \
-  X(16) X(18) X(20) X(24) X(27) X(30) X(32) X(36) X(40) X(45) X(48) X(50) X(54) \
-  X(60) X(64) X(72) X(80) X(81) X(90) X(96)
+  X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(45) X(48) X(50) X(54) \
+  X(60) X(64) X(72) X(75) X(80) X(81) X(90) X(96) X(100) X(108) X(120)
 
 template <typename T>
 static size_t team_lds_bytes(int n, int nxc) {
@@ -418,6 +418,8 @@ static size_t team_lds_bytes(int n, int nxc) {
 #define QEPW_TEAMS_OF(NN) if (n == NN) teams = TeamGeom<NN>::teams;
   QEPW_TEAM_SIZES(QEPW_TEAMS_OF)
 #undef QEPW_TEAMS_OF
+  // no team kernel for this length: the generic kernels run
+  if (teams == 0) return 0;
   // active columns, the zero column of k_planes_team, one scratch line per team
   return ((size_t)nxc + 1 + teams) * (size_t)(n + 1) * sizeof(cplx<T>);
 }

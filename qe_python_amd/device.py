@@ -247,6 +247,8 @@ class DeviceFFTPlan:
                                   device.index, ctypes.byref(handle))
         nat.check(rc, "qepw_plan_create")
         self.handle = handle
+        # FP64 needs the most LDS: a grid it supports runs in FP32 too
+        nat.check(lib.qepw_plan_supported(handle, nat.C128), "qepw_plan_supported")
         info = (ctypes.c_int64 * 8)()
         nat.check(lib.qepw_plan_query(handle, info), "qepw_plan_query")
         self.sticks = int(info[4])

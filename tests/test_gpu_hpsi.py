@@ -133,7 +133,9 @@ def test_team_kernel_sizes_against_oracle():
     from qe_python_amd.basis import make_bases
     from qe_python_amd.diagonalization import PlaneWaveHamiltonian
     rng = np.random.default_rng(23)
-    for n, n3, ecut in ((24, 24, 6.0), (30, 30, 10.0), (48, 45, 20.0), (90, 80, 60.0)):
+    for n, n3, ecut in ((24, 24, 6.0), (25, 25, 6.0), (30, 30, 10.0), (48, 45, 20.0),
+                        (75, 40, 40.0), (90, 80, 60.0), (100, 40, 48.0), (108, 40, 48.0),
+                        (120, 40, 48.0)):
         reciprocal = 2.0 * np.pi / 12.0 * np.eye(3)
         basis = make_bases(reciprocal, np.array([0.3, 0.1, 0.2]), ecut)[0]
         shape = (n, n, n3)
@@ -142,6 +144,35 @@ def test_team_kernel_sizes_against_oracle():
         ham = PlaneWaveHamiltonian(basis, np.zeros(shape, complex), (), real_potential=vr)
         ref = OracleHamiltonian(basis.indices, basis.kinetic, shape, vr).apply(x)
         assert rel(ham.apply(x), ref) <= 1e-12, n
+
+
+def test_square_planes_without_team_kernel_use_the_generic_kernel():
+    """12 x 12 and 15 x 15 planes (no team FFT) run the generic plane kernel."""
+    from qe_python_amd.basis import make_bases
+    from qe_python_amd.diagonalization import PlaneWaveHamiltonian
+    rng = np.random.default_rng(29)
+    for n, n3, ecut in ((12, 15, 1.0), (15, 12, 1.0)):
+        reciprocal = 2.0 * np.pi / 12.0 * np.eye(3)
+        basis = make_bases(reciprocal, np.array([0.2, 0.1, 0.0]), ecut)[0]
+        shape = (n, n, n3)
+        vr = rng.uniform(-1.0, 0.5, shape)
+        x = rng.standard_normal((len(basis), 2)) + 1j * rng.standard_normal((len(basis), 2))
+        ham = PlaneWaveHamiltonian(basis, np.zeros(shape, complex), (), real_potential=vr)
+        ref = OracleHamiltonian(basis.indices, basis.kinetic, shape, vr).apply(x)
+        assert rel(ham.apply(x), ref) <= 1e-12, n
+
+
+def test_unsupported_grid_fails_at_construction():
+    """A grid whose xy planes the kernels cannot hold in LDS is rejected when
+    the Hamiltonian is built, with a message naming the grid."""
+    from qe_python_amd.basis import make_bases
+    from qe_python_amd.diagonalization import PlaneWaveHamiltonian
+    reciprocal = 2.0 * np.pi / 12.0 * np.eye(3)
+    basis = make_bases(reciprocal, np.zeros(3), 6.0)[0]
+    shape = (160, 160, 24)
+    with pytest.raises(RuntimeError, match="160 x 160 x 24 grid is outside"):
+        PlaneWaveHamiltonian(basis, np.zeros(shape, complex), (),
+                             real_potential=np.zeros(shape))
 
 
 def test_hermiticity_on_device(small):
