@@ -79,7 +79,9 @@ void free_tables(qepw::StickTables& t) {
   hipFree(t.d_xcols);
   hipFree(t.d_stickq);
   hipFree(t.d_colmap);
+  hipFree(t.d_colstart);
   t.d_off = t.d_gpos = t.d_plane = t.d_xcols = t.d_stickq = t.d_colmap = nullptr;
+  t.d_colstart = nullptr;
 }
 
 void free_plan(qepw::Plan& p) {
@@ -96,7 +98,7 @@ void free_plan(qepw::Plan& p) {
 }
 
 struct TableHost {
-  std::vector<int> off, gpos, plane, xcols, stickq, colmap;
+  std::vector<int> off, gpos, plane, xcols, stickq, colmap, colstart;
 };
 
 hipError_t upload_tables(qepw::StickTables& t, const TableHost& h) {
@@ -106,6 +108,7 @@ hipError_t upload_tables(qepw::StickTables& t, const TableHost& h) {
   if ((e = upload(&t.d_plane, h.plane)) != hipSuccess) return e;
   if ((e = upload(&t.d_stickq, h.stickq)) != hipSuccess) return e;
   if ((e = upload(&t.d_colmap, h.colmap)) != hipSuccess) return e;
+  if ((e = upload(&t.d_colstart, h.colstart)) != hipSuccess) return e;
   return upload(&t.d_xcols, h.xcols);
 }
 
@@ -175,9 +178,11 @@ int qepw_plan_create(const int32_t* miller, int64_t npw, int n1, int n2, int n3,
   sphere.colmap.assign(n1, -1);
   for (size_t c = 0; c < sphere.xcols.size(); ++c) sphere.colmap[sphere.xcols[c]] = (int)c;
   sphere.stickq.resize(nst);
-  for (int s = 0; s < nst; ++s) {
+  sphere.colstart.assign(sphere.xcols.size() + 1, nst);
+  for (int s = nst - 1; s >= 0; --s) {
     const int x = sphere.plane[s] / n2, y = sphere.plane[s] % n2;
     sphere.stickq[s] = sphere.colmap[x] * n2 + y;
+    sphere.colstart[sphere.colmap[x]] = s;  // sticks are sorted by x, then y
   }
 
   TableHost full;
@@ -189,6 +194,10 @@ int qepw_plan_create(const int32_t* miller, int64_t npw, int n1, int n2, int n3,
   for (int s = 0; s < nfull; ++s) full.plane[s] = s;
   for (int64_t i = 0; i < ngrid; ++i) full.gpos[i] = (int)i;
   for (int x = 0; x < n1; ++x) full.xcols.push_back(x);
+  full.stickq.resize(nfull);
+  for (int s = 0; s < nfull; ++s) full.stickq[s] = s;  // column x = s / n2, y = s % n2
+  full.colstart.resize(n1 + 1);
+  for (int x = 0; x <= n1; ++x) full.colstart[x] = x * n2;
 
   qepw_plan* plan = new (std::nothrow) qepw_plan();
   if (!plan) return fail(QEPW_ERR_HIP, "qepw_plan_create: out of host memory");

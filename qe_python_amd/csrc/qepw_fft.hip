@@ -144,6 +144,168 @@ __global__ __launch_bounds__(kPlaneThreads) void k_planes(
   for (int s = tid; s < nst; s += kPlaneThreads) w[s] = P[plane_of[s]];
 }
 
+template <typename T> const cplx<T>* twiddles(const Plan& p, int axis);
+
+// Split plane stage for planes that no single workgroup can hold (large
+// FP64 grids): three passes through a per-band buffer T[z][c][y] of the
+// active columns' full y lines (n3 x nxc x n2 elements per band):
+//   k_split_cols_inverse : scatter each column's sticks into its y line,
+//                          backward y-FFT, store the line;
+//   k_split_rows         : gather row y from the columns, backward x-FFT,
+//                          times V (or, for the real-space potential, store
+//                          the real part and stop), forward x-FFT, scatter
+//                          back to the columns;
+//   k_split_cols_forward : forward y-FFT of each column, gather its sticks.
+// Every 1-D FFT is the Stockham LDS FFT of a group of lines.
+constexpr int kSplitThreads = 256;
+
+template <typename T>
+__global__ __launch_bounds__(kSplitThreads) void k_split_cols_inverse(
+    const cplx<T>* __restrict__ work, int64_t band_stride, int nst,
+    const int* __restrict__ colstart, const int* __restrict__ stick_q, int nxc, int n2,
+    cplx<T>* __restrict__ tbuf, int64_t tband, int group, LinePlan p2,
+    const cplx<T>* __restrict__ tw2) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* Ls = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * group;
+  const int ng = min(group, nxc - c0);
+  const int z = blockIdx.y;
+  const int64_t b = blockIdx.z;
+  const cplx<T> zero = mk<T>(T(0), T(0));
+  for (int i = tid; i < ng * n2; i += kSplitThreads) Ls[i] = zero;
+  __syncthreads();
+  const cplx<T>* src = work + b * band_stride + (int64_t)z * nst;
+  const int s0 = colstart[c0], s1 = colstart[c0 + ng];
+  for (int st = s0 + tid; st < s1; st += kSplitThreads) {
+    const int q = stick_q[st];  // c * n2 + y
+    Ls[q - c0 * n2] = src[st];
+  }
+  __syncthreads();
+  lds_fft<T, kSplitThreads, kPlanePoints>(Ls, ng, n2, 1, nullptr, p2, tw2, +1, tid);
+  cplx<T>* dst = tbuf + b * tband + ((int64_t)z * nxc + c0) * n2;
+  for (int i = tid; i < ng * n2; i += kSplitThreads) dst[i] = Ls[i];
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kSplitThreads) void k_split_rows(
+    cplx<T>* __restrict__ tbuf, int64_t tband, const int* __restrict__ xcols, int nxc, int n1,
+    int n2, const T* __restrict__ V, T* __restrict__ real_out, int group, LinePlan p1,
+    const cplx<T>* __restrict__ tw1) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* Ls = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int y0 = blockIdx.x * group;
+  const int nr = min(group, n2 - y0);
+  const int z = blockIdx.y;
+  const int64_t b = blockIdx.z;
+  const cplx<T> zero = mk<T>(T(0), T(0));
+  for (int i = tid; i < nr * n1; i += kSplitThreads) Ls[i] = zero;
+  __syncthreads();
+  cplx<T>* base = tbuf + b * tband + (int64_t)z * nxc * n2 + y0;
+  for (int i = tid; i < nr * nxc; i += kSplitThreads) {
+    const int r = i % nr, c = i / nr;
+    Ls[r * n1 + xcols[c]] = base[(int64_t)c * n2 + r];
+  }
+  __syncthreads();
+  lds_fft<T, kSplitThreads, kPlanePoints>(Ls, nr, n1, 1, nullptr, p1, tw1, +1, tid);
+  // V and the real output use the (n3, n2, n1) layout: (x, y) of plane z at
+  // (z * n2 + y) * n1 + x
+  const int64_t row0 = ((int64_t)z * n2 + y0) * n1;
+  if (MODE == kPlaneReal) {
+    for (int i = tid; i < nr * n1; i += kSplitThreads) real_out[row0 + i] = Ls[i].x;
+    return;
+  }
+  for (int i = tid; i < nr * n1; i += kSplitThreads) Ls[i] = cscale(Ls[i], V[row0 + i]);
+  __syncthreads();
+  lds_fft<T, kSplitThreads, kPlanePoints>(Ls, nr, n1, 1, nullptr, p1, tw1, -1, tid);
+  for (int i = tid; i < nr * nxc; i += kSplitThreads) {
+    const int r = i % nr, c = i / nr;
+    base[(int64_t)c * n2 + r] = Ls[r * n1 + xcols[c]];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSplitThreads) void k_split_cols_forward(
+    const cplx<T>* __restrict__ tbuf, int64_t tband, const int* __restrict__ colstart,
+    const int* __restrict__ stick_q, int nxc, int n2, cplx<T>* __restrict__ work,
+    int64_t band_stride, int nst, int group, LinePlan p2, const cplx<T>* __restrict__ tw2) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* Ls = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * group;
+  const int ng = min(group, nxc - c0);
+  const int z = blockIdx.y;
+  const int64_t b = blockIdx.z;
+  const cplx<T>* src = tbuf + b * tband + ((int64_t)z * nxc + c0) * n2;
+  for (int i = tid; i < ng * n2; i += kSplitThreads) Ls[i] = src[i];
+  __syncthreads();
+  lds_fft<T, kSplitThreads, kPlanePoints>(Ls, ng, n2, 1, nullptr, p2, tw2, -1, tid);
+  cplx<T>* dst = work + b * band_stride + (int64_t)z * nst;
+  const int s0 = colstart[c0], s1 = colstart[c0 + ng];
+  for (int st = s0 + tid; st < s1; st += kSplitThreads) {
+    const int q = stick_q[st];
+    dst[st] = Ls[q - c0 * n2];
+  }
+}
+
+// Lines per workgroup of the split stage for lines of plan lp (0: none fits).
+template <typename T>
+static int split_group(const LinePlan& lp) {
+  const int cap = lds_fft_capacity(lp, kSplitThreads, kPlanePoints) / lp.n;
+  const int by_lds = kMaxLdsBytes / (lp.n * (int)sizeof(cplx<T>));
+  int g = cap < by_lds ? cap : by_lds;
+  if (g > 16) g = 16;
+  return g > 0 ? g : 0;
+}
+
+template <typename T>
+static bool split_available(const Plan& p, const StickTables& st) {
+  return st.d_colstart && st.d_stickq && split_group<T>(p.line[0]) > 0 &&
+         split_group<T>(p.line[1]) > 0 && p.n1 <= kMaxLine && p.n2 <= kMaxLine;
+}
+
+// One band's T buffer of the split stage, in elements.
+static int64_t split_band_elems(const Plan& p, const StickTables& st) {
+  return (int64_t)p.n3 * st.nxcols * p.n2;
+}
+
+template <typename T>
+static void launch_split(bool real_mode, const Plan& p, const StickTables& st, cplx<T>* work,
+                         int64_t bstride, cplx<T>* tbuf, const T* V, T* real_out, int nb,
+                         hipStream_t stream) {
+  const int gc = split_group<T>(p.line[1]);
+  const int gr = split_group<T>(p.line[0]);
+  const size_t lc = (size_t)gc * p.n2 * sizeof(cplx<T>);
+  const size_t lr = (size_t)gr * p.n1 * sizeof(cplx<T>);
+  const int64_t tband = split_band_elems(p, st);
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_split_cols_inverse<T>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lc);
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_split_cols_forward<T>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lc);
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_split_rows<T, kPlaneApply>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr);
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_split_rows<T, kPlaneReal>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr);
+  const dim3 gcols((unsigned)((st.nxcols + gc - 1) / gc), (unsigned)p.n3, (unsigned)nb);
+  const dim3 grows((unsigned)((p.n2 + gr - 1) / gr), (unsigned)p.n3, (unsigned)nb);
+  hipLaunchKernelGGL(k_split_cols_inverse<T>, gcols, dim3(kSplitThreads), lc, stream, work,
+                     bstride, st.nst, st.d_colstart, st.d_stickq, st.nxcols, p.n2, tbuf, tband,
+                     gc, p.line[1], twiddles<T>(p, 1));
+  if (real_mode) {
+    hipLaunchKernelGGL((k_split_rows<T, kPlaneReal>), grows, dim3(kSplitThreads), lr, stream,
+                       tbuf, tband, st.d_xcols, st.nxcols, p.n1, p.n2, V, real_out, gr,
+                       p.line[0], twiddles<T>(p, 0));
+    return;
+  }
+  hipLaunchKernelGGL((k_split_rows<T, kPlaneApply>), grows, dim3(kSplitThreads), lr, stream,
+                     tbuf, tband, st.d_xcols, st.nxcols, p.n1, p.n2, V, real_out, gr, p.line[0],
+                     twiddles<T>(p, 0));
+  hipLaunchKernelGGL(k_split_cols_forward<T>, gcols, dim3(kSplitThreads), lc, stream, tbuf,
+                     tband, st.d_colstart, st.d_stickq, st.nxcols, p.n2, work, bstride, st.nst,
+                     gc, p.line[1], twiddles<T>(p, 1));
+}
+
 // Fast xy-plane kernel for square planes (n1 = n2 = N): the LDS holds only
 // the nxc active x columns of the plane, Q[c * NP + y], plus one all-zero
 // column; every 1-D FFT is a register "team" FFT (qepw_regfft.h).
@@ -516,11 +678,6 @@ static void set_lds_attrs(size_t stick_bytes, size_t plane_bytes) {
                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_bytes);
 }
 
-size_t apply_local_band_bytes(const Plan& p, int precision) {
-  const size_t c = precision == QEPW_C128 ? sizeof(double2) : sizeof(float2);
-  return (size_t)p.n3 * (size_t)p.sphere.nst * c;
-}
-
 template <typename T>
 static bool team_available(const Plan& p, const StickTables& st) {
   if (p.n1 != p.n2 || !st.d_stickq || !st.d_colmap) return false;
@@ -528,29 +685,60 @@ static bool team_available(const Plan& p, const StickTables& st) {
   return lds > 0 && lds <= (size_t)kMaxLdsBytes;
 }
 
+// How the xy-plane stage of a plan runs.
+enum PlanePath { kPathNone = 0, kPathTeam, kPathGeneric, kPathSplit };
+
+template <typename T>
+static bool generic_plane_fits(const Plan& p) {
+  const int np = p.n1 * p.n2;
+  return np * (int)sizeof(cplx<T>) <= kMaxLdsBytes &&
+         np <= lds_fft_capacity(p.line[0], kPlaneThreads, kPlanePoints) &&
+         np <= lds_fft_capacity(p.line[1], kPlaneThreads, kPlanePoints);
+}
+
+template <typename T>
+static int plane_path(const Plan& p, const StickTables& st) {
+  if (st.group * p.n3 > lds_fft_capacity(p.line[2], kStickThreads, kStickPts)) return kPathNone;
+  if (team_available<T>(p, st)) return kPathTeam;
+  if (generic_plane_fits<T>(p)) return kPathGeneric;
+  if (split_available<T>(p, st)) return kPathSplit;
+  return kPathNone;
+}
+
+// Workspace of one band: the stick buffer, plus the split stage's column
+// lines when that stage runs.
+template <typename T>
+static size_t band_bytes_t(const Plan& p, const StickTables& st) {
+  size_t e = (size_t)p.n3 * (size_t)st.nst;
+  if (plane_path<T>(p, st) == kPathSplit) e += (size_t)split_band_elems(p, st);
+  return e * sizeof(cplx<T>);
+}
+
+size_t apply_local_band_bytes(const Plan& p, int precision) {
+  return precision == QEPW_C128 ? band_bytes_t<double>(p, p.sphere)
+                                : band_bytes_t<float>(p, p.sphere);
+}
+
 template <typename T>
 int apply_local_t(const Plan& p, const void* vectors, int64_t ldv, int nb,
                   const void* potential, const void* kinetic, void* result,
                   int64_t ldr, void* workspace, size_t ws_bytes, hipStream_t stream) {
   const StickTables& st = p.sphere;
-  const bool team = team_available<T>(p, st);
-  if (st.group * p.n3 > lds_fft_capacity(p.line[2], kStickThreads, kStickPts))
-    return QEPW_ERR_UNSUPPORTED;
-  if (!team) {
-    int rc = check_fft_limits<T>(p, st);
-    if (rc) return rc;
-  }
+  const int path = plane_path<T>(p, st);
+  if (path == kPathNone) return QEPW_ERR_UNSUPPORTED;
   if (nb <= 0) return QEPW_OK;
-  const size_t band_bytes = (size_t)p.n3 * st.nst * sizeof(cplx<T>);
+  const size_t band_bytes = band_bytes_t<T>(p, st);
   int chunk = (int)(ws_bytes / band_bytes);
   if (chunk < 1) return QEPW_ERR_WORKSPACE;
   if (chunk > 65535) chunk = 65535;
   const size_t stick_lds = (size_t)st.group * (p.n3 + 1) * sizeof(cplx<T>);
   const size_t plane_lds = (size_t)p.n1 * p.n2 * sizeof(cplx<T>);
-  set_lds_attrs<T>(stick_lds, team ? 0 : plane_lds);
+  set_lds_attrs<T>(stick_lds, path == kPathGeneric ? plane_lds : 0);
   const int64_t bstride = (int64_t)p.n3 * st.nst;
   const T inv_n = T(1.0 / ((double)p.n1 * p.n2 * p.n3));
   cplx<T>* work = static_cast<cplx<T>*>(workspace);
+  // the split stage's column lines follow the chunk's stick buffers
+  cplx<T>* tbuf = work + (int64_t)chunk * bstride;
   for (int b0 = 0; b0 < nb; b0 += chunk) {
     const int nbc = (nb - b0) < chunk ? (nb - b0) : chunk;
     const cplx<T>* src = static_cast<const cplx<T>*>(vectors) + (int64_t)b0 * ldv;
@@ -561,8 +749,12 @@ int apply_local_t(const Plan& p, const void* vectors, int64_t ldv, int nb,
       hipLaunchKernelGGL(k_sticks_inverse<T>, gs, dim3(kStickThreads), stick_lds, stream,
                          src, ldv, work, bstride, st.nst, p.n3, st.d_off, st.d_gpos,
                          st.group, p.line[2], twiddles<T>(p, 2));
-    if (!team || !launch_planes_team<T>(p, st, work, bstride, static_cast<const T*>(potential),
-                                        twiddles<T>(p, 0), nbc, stream))
+    if (path == kPathSplit)
+      launch_split<T>(false, p, st, work, bstride, tbuf, static_cast<const T*>(potential),
+                      nullptr, nbc, stream);
+    else if (path != kPathTeam ||
+             !launch_planes_team<T>(p, st, work, bstride, static_cast<const T*>(potential),
+                                    twiddles<T>(p, 0), nbc, stream))
       hipLaunchKernelGGL((k_planes<T, kPlaneApply>), gp, dim3(kPlaneThreads), plane_lds,
                          stream, work, bstride, st.nst, st.d_plane, st.d_xcols, st.nxcols,
                          static_cast<const T*>(potential), static_cast<T*>(nullptr), p.n1,
@@ -819,7 +1011,9 @@ int accumulate_density(const Plan& p, int precision, const void* vectors, int64_
 }
 
 size_t inverse_real_bytes(const Plan& p) {
-  return (size_t)p.n1 * p.n2 * p.n3 * sizeof(double2);
+  const size_t grid = (size_t)p.n1 * p.n2 * p.n3 * sizeof(double2);
+  // the split stage adds one line buffer of the whole grid
+  return plane_path<double>(p, p.full) == kPathSplit ? 2 * grid : grid;
 }
 
 // V(r) = Re backward-FFT(V(G)) of a C-order (n1, n2, n3) complex grid, written
@@ -828,23 +1022,27 @@ size_t inverse_real_bytes(const Plan& p) {
 int inverse_real(const Plan& p, const void* grid_g, void* out_real, void* workspace,
                  size_t ws_bytes, hipStream_t stream) {
   const StickTables& st = p.full;
-  int rc = check_fft_limits<double>(p, st);
-  if (rc) return rc;
+  const int path = plane_path<double>(p, st);
+  if (path == kPathNone) return QEPW_ERR_UNSUPPORTED;
   if (ws_bytes < inverse_real_bytes(p)) return QEPW_ERR_WORKSPACE;
   const size_t stick_lds = (size_t)st.group * (p.n3 + 1) * sizeof(double2);
   const size_t plane_lds = (size_t)p.n1 * p.n2 * sizeof(double2);
-  set_lds_attrs<double>(stick_lds, plane_lds);
+  set_lds_attrs<double>(stick_lds, path == kPathSplit ? 0 : plane_lds);
   const int64_t bstride = (int64_t)p.n3 * st.nst;
   double2* work = static_cast<double2*>(workspace);
   hipLaunchKernelGGL(k_sticks_inverse<double>, dim3(st.ngroups, 1), dim3(kStickThreads),
                      stick_lds, stream, static_cast<const double2*>(grid_g), (int64_t)0, work,
                      bstride, st.nst, p.n3, st.d_off, st.d_gpos, st.group, p.line[2],
                      twiddles<double>(p, 2));
-  hipLaunchKernelGGL((k_planes<double, kPlaneReal>), dim3(p.n3, 1), dim3(kPlaneThreads),
-                     plane_lds, stream, work, bstride, st.nst, st.d_plane, st.d_xcols,
-                     st.nxcols, static_cast<const double*>(nullptr),
-                     static_cast<double*>(out_real), p.n1, p.n2, p.line[0], p.line[1],
-                     twiddles<double>(p, 0), twiddles<double>(p, 1));
+  if (path == kPathSplit)
+    launch_split<double>(true, p, st, work, bstride, work + bstride, nullptr,
+                         static_cast<double*>(out_real), 1, stream);
+  else
+    hipLaunchKernelGGL((k_planes<double, kPlaneReal>), dim3(p.n3, 1), dim3(kPlaneThreads),
+                       plane_lds, stream, work, bstride, st.nst, st.d_plane, st.d_xcols,
+                       st.nxcols, static_cast<const double*>(nullptr),
+                       static_cast<double*>(out_real), p.n1, p.n2, p.line[0], p.line[1],
+                       twiddles<double>(p, 0), twiddles<double>(p, 1));
   return hipGetLastError() == hipSuccess ? QEPW_OK : QEPW_ERR_HIP;
 }
 

@@ -30,6 +30,8 @@ struct StickTables {
   int* d_xcols = nullptr;  // nxcols: active x values
   int* d_stickq = nullptr;  // nst: c * n2 + y, c = rank of the stick's x in xcols
   int* d_colmap = nullptr;  // n1: rank of x among active columns, or -1
+  int* d_colstart = nullptr;  // nxcols+1: first stick of each active column (sticks
+                              // are sorted by x, then y)
   unsigned slot_mask = 0;   // team-FFT input slots holding an active x or stick y
 };
 

@@ -162,15 +162,39 @@ def test_square_planes_without_team_kernel_use_the_generic_kernel():
         assert rel(ham.apply(x), ref) <= 1e-12, n
 
 
+@pytest.mark.parametrize("shape", [(128, 128, 20), (144, 150, 16)])
+def test_large_planes_split_stage(shape):
+    """Planes no workgroup can hold in FP64 (128 x 128, and a non-square
+    144 x 150) take the split plane stage: h_psi in FP64 and FP32 against the
+    oracle, and the device V(G) -> V(r) transform against numpy."""
+    from qe_python_amd.basis import make_bases
+    from qe_python_amd.diagonalization import PlaneWaveHamiltonian
+    rng = np.random.default_rng(31)
+    reciprocal = 2.0 * np.pi / 12.0 * np.eye(3)
+    basis = make_bases(reciprocal, np.array([0.1, 0.3, 0.2]), 8.0)[0]
+    vr = rng.uniform(-1.0, 0.5, shape)
+    x = rng.standard_normal((len(basis), 3)) + 1j * rng.standard_normal((len(basis), 3))
+    ref = OracleHamiltonian(basis.indices, basis.kinetic, shape, vr).apply(x)
+    ham = PlaneWaveHamiltonian(basis, np.zeros(shape, complex), (), real_potential=vr)
+    assert rel(ham.apply(x), ref) <= 1e-12
+    ham32 = PlaneWaveHamiltonian(basis, np.zeros(shape, complex), (), real_potential=vr,
+                                 precision="fp32")
+    assert rel(ham32.apply(x), ref) <= 2e-6
+    vg = np.fft.fftn(vr) / vr.size
+    ham_g = PlaneWaveHamiltonian(basis, vg)
+    np.testing.assert_allclose(ham_g.real_potential, vr, atol=1e-13, rtol=0)
+
+
 def test_unsupported_grid_fails_at_construction():
-    """A grid whose xy planes the kernels cannot hold in LDS is rejected when
-    the Hamiltonian is built, with a message naming the grid."""
+    """FFT lines longer than 256 points are rejected when the Hamiltonian is
+    built, with a message naming the limit (every grid up to 256 per axis
+    runs: team, generic or split plane stage)."""
     from qe_python_amd.basis import make_bases
     from qe_python_amd.diagonalization import PlaneWaveHamiltonian
     reciprocal = 2.0 * np.pi / 12.0 * np.eye(3)
     basis = make_bases(reciprocal, np.zeros(3), 6.0)[0]
-    shape = (160, 160, 24)
-    with pytest.raises(RuntimeError, match="160 x 160 x 24 grid is outside"):
+    shape = (270, 270, 24)
+    with pytest.raises(RuntimeError, match="<= 256"):
         PlaneWaveHamiltonian(basis, np.zeros(shape, complex), (),
                              real_potential=np.zeros(shape))
 
