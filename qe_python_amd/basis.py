@@ -11,8 +11,12 @@ order, and therefore the same Davidson storage layout, as the reference's:
   1456-1470 and ``_miller_grid`` 1473-1478)
 * ``fft_shape``           -> ``basis.py:1619-1682``
 * ``potential_matrix``    -> ``basis.py:1685-1688``
+* ``LocalPotentialWorkspace`` -> ``basis.py:457-555, 1210-1282`` (the device FFT
+  plan of one k-point and the SCF's density accumulation)
 
-Nothing here runs per Hamiltonian application.
+Apart from ``LocalPotentialWorkspace`` (which holds device state and imports
+the device modules when it is built), nothing here touches the GPU or runs per
+Hamiltonian application.
 """
 
 from __future__ import annotations
@@ -171,3 +175,94 @@ def linear_slots(indices: np.ndarray, shape) -> np.ndarray:
     idx = np.asarray(indices)
     return np.ravel_multi_index(tuple(idx[:, a] % shape[a] for a in range(3)),
                                 tuple(shape)).astype(np.int64)
+
+
+class LocalPotentialWorkspace:
+    """Device counterpart of the reference's ``LocalPotentialWorkspace``
+    (``qepy_pw/basis.py:457-555``, serial branch) for one k-point basis: the
+    sphere-pruned FFT plan on the GPU, shared by a ``PlaneWaveHamiltonian``
+    built with ``local_workspace=`` this object, and the occupied-state
+    density of the SCF's ``sum_band`` (``accumulate_density``,
+    ``basis.py:1210-1282``; called from ``pw/scf.py:1382-1451``).
+
+    Only the single-process form exists here (one GPU): an ``mpi`` of more
+    than one rank is rejected, as the device Hamiltonian does.
+    """
+
+    def __init__(self, indices, shape, *, mpi=None, timers=None, device=None, **options):
+        from . import device as dv
+        idx = indices.indices if hasattr(indices, "indices") else indices
+        idx = np.asarray(idx)
+        if idx.ndim != 2 or idx.shape[1] != 3:
+            raise ValueError("indices must have shape (nplane, 3)")
+        self.shape = tuple(int(s) for s in shape)
+        if len(self.shape) != 3:
+            raise ValueError("the FFT grid shape must be three-dimensional")
+        if mpi is not None and getattr(mpi, "size", 1) > 1:
+            raise ValueError("the device workspace runs on one GPU; a distributed "
+                             "LocalPotentialWorkspace is not supported")
+        self.mpi = mpi
+        self.timers = timers
+        self.local_slab = slice(0, self.shape[2])
+        self.device = dv.require_device(device)
+        self.indices = idx
+        self.plan = dv.DeviceFFTPlan(idx, self.shape, self.device)
+        self.workspace = dv.Workspace(self.device)
+        self.npw = self.plan.npw
+
+    def accumulate_density(self, density, coefficients, band_weights) -> None:
+        """density += sum_b band_weights[b] |psi_b(r)|^2, psi_b(r) the
+        unnormalised backward FFT of coefficients[:, b] (reference row order).
+        ``density``: (n1, n2, n3) float64, a numpy array or a device tensor,
+        updated in place; ``coefficients``: (npw, nb) complex, host or
+        device."""
+        import torch
+
+        from . import _native as nat
+        from . import device as dv
+        weights = np.asarray(band_weights, dtype=float)
+        nb = len(weights)
+        if getattr(coefficients, "ndim", 0) != 2 or coefficients.shape[1] != nb:
+            raise ValueError("density coefficients and band weights disagree")
+        if tuple(density.shape) != self.shape:
+            raise ValueError("density accumulator has the wrong local shape")
+        if coefficients.shape[0] != self.npw:
+            raise ValueError("density coefficients do not match the basis")
+        if nb == 0:
+            return
+        context = self.timers.measure("fftw") if self.timers is not None else None
+        if context is not None:
+            context.__enter__()
+        try:
+            from .diagonalization import _upload_block
+            # device order, one band per row of a (nb, npw) tensor
+            psi = _upload_block(coefficients, self.plan.perm_device, self.npw, self.device,
+                                torch.complex128)
+            w_dev = dv.upload(weights, self.device)
+            n1, n2, n3 = self.shape
+            out = torch.empty((n3, n2, n1), dtype=torch.float64, device=self.device)
+            self._accumulate_device(psi, w_dev, out)
+            grid = out.permute(2, 1, 0)
+            if isinstance(density, torch.Tensor):
+                density += grid.to(density.device, density.dtype)
+            else:
+                density += grid.cpu().numpy()
+        finally:
+            if context is not None:
+                context.__exit__(None, None, None)
+
+    def _accumulate_device(self, psi, weights, out) -> None:
+        """out (n3, n2, n1) = sum_b weights[b] |psi_b(r)|^2 for device-order
+        bands psi (nb, npw) and device weights (nb), on the current stream."""
+        from . import _native as nat
+        from . import device as dv
+        lib = nat.load()
+        prec = dv.precision_code(psi.dtype)
+        nb = int(psi.shape[0])
+        need = lib.qepw_accumulate_density_workspace(self.plan.handle, prec, nb)
+        buf = self.workspace.get(need)
+        rc = lib.qepw_accumulate_density(self.plan.handle, prec, psi.data_ptr(), dv.ld_of(psi),
+                                         nb, weights.data_ptr(), 1.0, out.data_ptr(),
+                                         buf.data_ptr(), buf.numel(),
+                                         dv.stream_handle(self.device))
+        nat.check(rc, "qepw_accumulate_density")

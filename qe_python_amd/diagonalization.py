@@ -220,7 +220,14 @@ class PlaneWaveHamiltonian:
         self.potential_average = (float(potential_average) if potential_average is not None
                                   else float(np.real(self.potential_g.flat[0])))
         self.projector_terms = tuple(projector_terms)
-        self.plan = dv.DeviceFFTPlan(indices, shape, self.device)
+        shared = getattr(local_workspace, "plan", None)
+        if (isinstance(shared, dv.DeviceFFTPlan) and shared.device == self.device
+                and shared.shape == shape and shared.npw == self.npw
+                and np.array_equal(getattr(local_workspace, "indices", None), indices)):
+            # a device LocalPotentialWorkspace of this basis: share its plan
+            self.plan = shared
+        else:
+            self.plan = dv.DeviceFFTPlan(indices, shape, self.device)
         self.workspace = dv.Workspace(self.device)
         perm = self.plan.perm
         if real_potential is not None:
