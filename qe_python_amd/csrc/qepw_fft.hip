@@ -790,22 +790,26 @@ int apply_local(const Plan& p, int precision, const void* vectors, int64_t ldv, 
 // LDS / registers, writing one partial plane per (slice, z); a last kernel
 // sums the slices in a fixed order.  No atomics: results are reproducible.
 
-template <typename T, int N>
+template <typename T, int N, bool PRUNE>
 __global__ __launch_bounds__(kTeamThreads) void k_density_team(
     const cplx<T>* __restrict__ work, int64_t band_stride, int nst,
     const int* __restrict__ stick_q, const int* __restrict__ colmap, int nxc, int nb,
     int bands_per_slice, const double* __restrict__ weights, double* __restrict__ partial,
     int accumulate, const cplx<T>* __restrict__ tw) {
   using G = TeamGeom<N>;
+  constexpr int L = G::L, M = G::M;
+  constexpr unsigned KEEP = PRUNE ? G::pruned : G::all;
+  // rows of a team: y = team + r * teams, r < R; each lane keeps the density
+  // of its outputs x = t + L k of those rows in registers across the bands
+  constexpr int R = (N + G::teams - 1) / G::teams;
   extern __shared__ unsigned char smem_raw[];
-  constexpr int NP = N + 1;  // padded column stride, as in k_planes_team
+  constexpr int NP = N + 1;  // padded column stride and zero column, as in k_planes_team
   cplx<T>* Q = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* scratch_all = Q + nxc * NP;
-  double* D = reinterpret_cast<double*>(scratch_all + G::teams * NP);
+  cplx<T>* scratch_all = Q + (nxc + 1) * NP;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int team_in_wave = lane / G::M;
-  const int t = lane - team_in_wave * G::M;
+  const int team_in_wave = lane / M;
+  const int t = lane - team_in_wave * M;
   const bool in_team = team_in_wave < G::per_wave;
   const int team = wave * G::per_wave + team_in_wave;
   cplx<T>* S = scratch_all + team * NP;
@@ -813,14 +817,21 @@ __global__ __launch_bounds__(kTeamThreads) void k_density_team(
   const int slice = blockIdx.y;
   const int b0 = slice * bands_per_slice;
   const int b1 = min(nb, b0 + bands_per_slice);
-  const int nq = nxc * NP;
+  const int nq = (nxc + 1) * NP;
   const cplx<T> zero = mk<T>(T(0), T(0));
   TeamTwiddles<T, N> twr;
   twr.load(t, tw);
-  int cin[G::M];
+  int cq[L];
 #pragma unroll
-  for (int s = 0; s < G::M; ++s) cin[s] = t < G::L ? colmap[t + G::L * s] : -1;
-  for (int i = tid; i < N * N; i += kTeamThreads) D[i] = 0.0;
+  for (int s = 0; s < L; ++s) {
+    const int c = (t < M && ((KEEP >> s) & 1u)) ? colmap[t + M * s] : -1;
+    cq[s] = (c < 0 ? nxc : c) * NP;
+  }
+  double acc[R][M];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int k = 0; k < M; ++k) acc[r][k] = 0.0;
   for (int b = b0; b < b1; ++b) {
     const cplx<T>* w = work + (int64_t)b * band_stride + (int64_t)z * nst;
     const double wb = weights[b];
@@ -831,29 +842,68 @@ __global__ __launch_bounds__(kTeamThreads) void k_density_team(
       Q[q + q / N] = w[s];
     }
     __syncthreads();
+    // backward y-FFT of every active column (in place), inputs pruned
     if (in_team) {
       for (int c = team; c < nxc; c += G::teams) {
+        if (c != team) wave_lds_sync();
         cplx<T>* col = Q + c * NP;
-        team_fft<T, N, +1>(t, S, twr, [&](int j, int) { return col[j]; },
-                           [&](int k, int, cplx<T> v) { col[k] = v; });
+        cplx<T> bb[L], a[M];
+        if (t < M) {
+#pragma unroll
+          for (int s = 0; s < L; ++s) bb[s] = ((KEEP >> s) & 1u) ? col[t + M * s] : zero;
+        }
+        dif_front<T, N, +1>(t, S, twr, bb);
+        wave_lds_sync();
+        dif_back<T, N, +1>(t, S, a);
+        if (t < L) {
+#pragma unroll
+          for (int k = 0; k < M; ++k) col[t + L * k] = a[k];
+        }
       }
     }
     __syncthreads();
+    // backward x-FFT of this team's rows, |psi|^2 accumulated in registers
     if (in_team) {
-      for (int y = team; y < N; y += G::teams) {
-        double* drow = D + y * N;
-        team_fft<T, N, +1>(
-            t, S, twr,
-            [&](int, int s) { return cin[s] >= 0 ? Q[cin[s] * NP + y] : zero; },
-            [&](int k, int, cplx<T> v) {
-              drow[k] += wb * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
-            });
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int y = team + r * G::teams;
+        if (y < N) {
+          if (r > 0) wave_lds_sync();
+          cplx<T> bb[L], a[M];
+          if (t < M) {
+#pragma unroll
+            for (int s = 0; s < L; ++s)
+              bb[s] = ((KEEP >> s) & 1u) ? Q[cq[s] + y] : zero;
+          }
+          dif_front<T, N, +1>(t, S, twr, bb);
+          wave_lds_sync();
+          dif_back<T, N, +1>(t, S, a);
+          if (t < L) {
+#pragma unroll
+            for (int k = 0; k < M; ++k)
+              acc[r][k] += wb * ((double)a[k].x * (double)a[k].x +
+                                 (double)a[k].y * (double)a[k].y);
+          }
+        }
       }
     }
     __syncthreads();
   }
+  // (y, x) of the plane, (n3, n2, n1) layout: one writer per element
   double* out = partial + ((int64_t)slice * gridDim.x + z) * N * N;
-  for (int i = tid; i < N * N; i += kTeamThreads) out[i] = (accumulate ? out[i] : 0.0) + D[i];
+  if (in_team && t < L) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int y = team + r * G::teams;
+      if (y < N) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          const int i = y * N + t + L * k;
+          out[i] = (accumulate ? out[i] : 0.0) + acc[r][k];
+        }
+      }
+    }
+  }
 }
 
 template <typename T>
@@ -926,8 +976,8 @@ size_t density_workspace_bytes(const Plan& p, int precision, int nb) {
 
 template <typename T>
 static size_t density_team_lds(int n, int nxc) {
-  const size_t base = team_lds_bytes<T>(n, nxc);
-  return base == 0 ? 0 : base + (size_t)n * n * sizeof(double);
+  // the density accumulates in registers: the plane kernel's LDS
+  return team_lds_bytes<T>(n, nxc);
 }
 
 template <typename T>
@@ -975,17 +1025,25 @@ int accumulate_density_t(const Plan& p, const void* vectors, int64_t ldv, int nb
     const int acc = b0 > 0 ? 1 : 0;
     bool launched = false;
     if (team) {
+#define QEPW_LAUNCH_DENSITY_K(NN, PR)                                                        \
+  do {                                                                                       \
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density_team<T, NN, PR>),          \
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)team_lds);          \
+    hipLaunchKernelGGL((k_density_team<T, NN, PR>), grid, dim3(kTeamThreads), team_lds,     \
+                       stream, work, bstride, st.nst, st.d_stickq, st.d_colmap, st.nxcols,  \
+                       nbc, per, weights + b0, partial, acc, twiddles<T>(p, 0));             \
+  } while (0)
 #define QEPW_LAUNCH_DENSITY(NN)                                                              \
   if (!launched && p.n1 == NN) {                                                             \
-    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density_team<T, NN>),              \
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)team_lds);          \
-    hipLaunchKernelGGL((k_density_team<T, NN>), grid, dim3(kTeamThreads), team_lds, stream, \
-                       work, bstride, st.nst, st.d_stickq, st.d_colmap, st.nxcols, nbc, per, \
-                       weights + b0, partial, acc, twiddles<T>(p, 0));                       \
+    if ((st.slot_mask & ~TeamGeom<NN>::pruned) == 0u)                                        \
+      QEPW_LAUNCH_DENSITY_K(NN, true);                                                       \
+    else                                                                                     \
+      QEPW_LAUNCH_DENSITY_K(NN, false);                                                      \
     launched = true;                                                                         \
   }
       QEPW_TEAM_SIZES(QEPW_LAUNCH_DENSITY)
 #undef QEPW_LAUNCH_DENSITY
+#undef QEPW_LAUNCH_DENSITY_K
     }
     if (!launched)
       hipLaunchKernelGGL(k_density_plane<T>, grid, dim3(kPlaneThreads), plane_lds, stream, work,
