@@ -240,6 +240,25 @@ int qepw_build_projectors(int precision, const void* beta, int64_t ldb, int nch,
                           int64_t ldo, int64_t nrows, void* stream);
 
 /*
+ * beta[g, j] = table[i, p_j] interpolated at |q_g| times Y_{l_j m_j}(q_g)
+ * (-i)^{l_j}: the per-k assembly of one species' projector channels.
+ * q: device double (npw, 3) Cartesian k+G (any row order; beta follows it);
+ * table: device double, column-major (nq x nradial, leading dimension ldt),
+ * radial transforms on the uniform grid i * dq, interpolated with QE's
+ * four-point cubic formula (samples i0 .. i0+3, i0 = floor(|q| / dq)): the
+ * table must extend to floor(max |q| / dq) + 3; chan_radial, chan_l, chan_m:
+ * device int32 (nch), radial index, l <= 3 and QE ylmr2 index m (0, then
+ * cos and sin of 1..l); beta: device complex128 (npw x nch, column-major).
+ * Replaces: the per-k beta assembly of UPFPseudo.projector_basis
+ *   (qepy_pw/upf.py:483-541) with _qe_cubic_interpolate (upf.py:880-912)
+ *   and _qe_real_spherical_harmonics (upf.py:722-772) -- QE init_us_2; the
+ *   per-atom phases are qepw_build_projectors.
+ */
+int qepw_projector_basis(const double* q, int64_t npw, const double* table, int64_t ldt,
+                         int nq, double dq, const int32_t* chan_radial, const int32_t* chan_l,
+                         const int32_t* chan_m, int nch, void* beta, int64_t ldb, void* stream);
+
+/*
  * out[a, c] = 2 sum_n f_n sum_{i in atom a} Re(conj(dO_c[i, n]) DO[i, n]):
  * dE/dtau (the negative force) of the nonlocal energy, with dO_c = dO/dtau_c
  * (three (P, nb) blocks, block stride dstride), DO = D O, atom a owning

@@ -69,6 +69,9 @@ SIGNATURES = [
                                 c_int64, c_int64, c_int, c_void_p]),
     ("qepw_build_projectors", c_int, [c_int, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_int,
                                       c_void_p, c_int64, c_int64, c_void_p]),
+    ("qepw_projector_basis", c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int, c_double,
+                                     c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int64,
+                                     c_void_p]),
     ("qepw_projector_forces", c_int, [c_int, c_void_p, c_int64, c_int64, c_void_p, c_int64,
                                       c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     ("qepw_hermitian_complete", c_int, [c_void_p, c_int64, c_int, c_int, c_void_p]),

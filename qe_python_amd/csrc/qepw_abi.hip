@@ -486,6 +486,19 @@ int qepw_build_projectors(int precision, const void* beta, int64_t ldb, int nch,
                 "qepw_build_projectors");
 }
 
+int qepw_projector_basis(const double* q, int64_t npw, const double* table, int64_t ldt,
+                         int nq, double dq, const int32_t* chan_radial, const int32_t* chan_l,
+                         const int32_t* chan_m, int nch, void* beta, int64_t ldb, void* stream) {
+  if (npw < 0 || nch < 0) return fail(QEPW_ERR_ARG, "qepw_projector_basis: negative size");
+  if (npw == 0 || nch == 0) return QEPW_OK;
+  if (!q || !table || !chan_radial || !chan_l || !chan_m || !beta || ldb < npw || nq < 4 ||
+      ldt < nq || !(dq > 0.0))
+    return fail(QEPW_ERR_ARG, "qepw_projector_basis: bad argument");
+  return finish(qepw::projector_basis(q, npw, table, ldt, nq, dq, chan_radial, chan_l, chan_m,
+                                      nch, beta, ldb, static_cast<hipStream_t>(stream)),
+                "qepw_projector_basis");
+}
+
 int qepw_projector_forces(int precision, const void* dO, int64_t lddo, int64_t dstride,
                           const void* DO, int64_t lddO, const double* f, int nb,
                           const int32_t* off, int natom, double* out, void* stream) {

@@ -63,6 +63,10 @@ int projector_forces(int precision, const void* dO, int64_t lddo, int64_t dstrid
                      const void* DO, int64_t lddO, const double* f, int nb, const int* off,
                      int natom, double* out, hipStream_t stream);
 
+int projector_basis(const double* q, int64_t npw, const double* table, int64_t ldt, int nq,
+                    double dq, const int* chan_radial, const int* chan_l, const int* chan_m,
+                    int nch, void* beta, int64_t ldb, hipStream_t stream);
+
 // qepw_eig.hip
 int hermitian_complete(void* a, int64_t lda, int first, int active, hipStream_t stream);
 int shift_scale(const void* a, int64_t lda, int n, double sigma, void* x, int64_t ldx,

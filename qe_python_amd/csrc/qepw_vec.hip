@@ -365,6 +365,96 @@ int scale_rows(int precision, const void* src, int64_t lds, const double* s, int
   return status();
 }
 
+// Real spherical harmonic of QE ylmr2 order (m = 0, then cos and sin of
+// m = 1..l; Condon-Shortley phase) at the unit direction (x, y, z), l <= 3.
+__device__ __forceinline__ double qe_real_harmonic(int l, int m, double x, double y, double z) {
+  const double pi = 3.14159265358979323846;
+  switch (l) {
+    case 0:
+      return 1.0 / sqrt(4.0 * pi);
+    case 1: {
+      const double c = sqrt(3.0 / (4.0 * pi));
+      return m == 0 ? c * z : (m == 1 ? -c * x : -c * y);
+    }
+    case 2: {
+      const double a = sqrt(15.0 / (4.0 * pi));
+      switch (m) {
+        case 0: return sqrt(5.0 / (16.0 * pi)) * (3.0 * z * z - 1.0);
+        case 1: return -a * x * z;
+        case 2: return -a * y * z;
+        case 3: return sqrt(15.0 / (16.0 * pi)) * (x * x - y * y);
+        default: return a * x * y;
+      }
+    }
+    default: {
+      switch (m) {
+        case 0: return 0.25 * sqrt(7.0 / pi) * (5.0 * z * z * z - 3.0 * z);
+        case 1: return -0.125 * sqrt(42.0 / pi) * (5.0 * z * z - 1.0) * x;
+        case 2: return -0.125 * sqrt(42.0 / pi) * (5.0 * z * z - 1.0) * y;
+        case 3: return 0.25 * sqrt(105.0 / pi) * z * (x * x - y * y);
+        case 4: return 0.5 * sqrt(105.0 / pi) * x * y * z;
+        case 5: return -0.125 * sqrt(70.0 / pi) * (x * x * x - 3.0 * x * y * y);
+        default: return -0.125 * sqrt(70.0 / pi) * (3.0 * x * x * y - y * y * y);
+      }
+    }
+  }
+}
+
+// beta[g, j] = table_{p_j}(|q_g|) Y_{l_j m_j}(q_g) (-i)^{l_j}: the per-k
+// assembly of the species projectors (QE init_us_2; reference
+// qepy_pw/upf.py:483-541) with QE's four-point cubic interpolation of the
+// radial table (upf.py:880-912): samples i0 .. i0+3, i0 = floor(|q| / dq).
+// The +z direction stands for q = 0.
+__global__ void k_projector_basis(const double* __restrict__ q, int64_t npw,
+                                  const double* __restrict__ table, int64_t ldt, int nq,
+                                  double dq, const int* __restrict__ chan_radial,
+                                  const int* __restrict__ chan_l, const int* __restrict__ chan_m,
+                                  double2* __restrict__ beta, int64_t ldb) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (g >= npw) return;
+  const double qx = q[3 * g], qy = q[3 * g + 1], qz = q[3 * g + 2];
+  const double qn = sqrt(qx * qx + qy * qy + qz * qz);
+  double x = 0.0, y = 0.0, z = 1.0;
+  if (qn > 1.0e-14) {
+    x = qx / qn;
+    y = qy / qn;
+    z = qz / qn;
+  }
+  const double scaled = qn / dq;
+  int lower = (int)floor(scaled);
+  // the host checks that the table covers every |q|; the clamp only keeps
+  // the reads inside it
+  lower = lower < 0 ? 0 : (lower > nq - 4 ? nq - 4 : lower);
+  const double px = scaled - lower;
+  const double ux = 1.0 - px, vx = 2.0 - px, wx = 3.0 - px;
+  const double* t = table + (int64_t)chan_radial[j] * ldt + lower;
+  double radial = t[0] * (ux * vx * wx / 6.0);
+  radial += t[1] * (px * vx * wx / 2.0);
+  radial += t[2] * (-px * ux * wx / 2.0);
+  radial += t[3] * (px * ux * vx / 6.0);
+  const int l = chan_l[j];
+  const double v = radial * qe_real_harmonic(l, chan_m[j], x, y, z);
+  double2 out;
+  switch (l & 3) {  // (-i)^l
+    case 0: out = make_double2(v, 0.0); break;
+    case 1: out = make_double2(0.0, -v); break;
+    case 2: out = make_double2(-v, 0.0); break;
+    default: out = make_double2(0.0, v); break;
+  }
+  beta[g + (int64_t)j * ldb] = out;
+}
+
+int projector_basis(const double* q, int64_t npw, const double* table, int64_t ldt, int nq,
+                    double dq, const int* chan_radial, const int* chan_l, const int* chan_m,
+                    int nch, void* beta, int64_t ldb, hipStream_t stream) {
+  if (npw <= 0 || nch <= 0) return QEPW_OK;
+  const dim3 grid((unsigned)((npw + 255) / 256), (unsigned)nch);
+  hipLaunchKernelGGL(k_projector_basis, grid, dim3(256), 0, stream, q, npw, table, ldt, nq, dq,
+                     chan_radial, chan_l, chan_m, static_cast<double2*>(beta), ldb);
+  return hipGetLastError() == hipSuccess ? QEPW_OK : QEPW_ERR_HIP;
+}
+
 int build_projectors(int precision, const void* beta, int64_t ldb, int nch, const double* q,
                      const double* tau, int natom, void* out, int64_t ldo, int64_t nrows,
                      hipStream_t stream) {

@@ -261,7 +261,10 @@ class PlaneWaveHamiltonian:
         for term in self.projector_terms:
             if isinstance(term, PackedProjectorTerm) or (
                     hasattr(term, "diagonal") and hasattr(term, "beta")):
-                blocks.append(np.asarray(term.beta))
+                # a device-built packed term (qe_python_amd.projectors) stays
+                # on the device
+                blocks.append(term.beta if isinstance(term.beta, torch.Tensor)
+                              else np.asarray(term.beta))
                 couplings.append(np.asarray(term.coupling))
             elif isinstance(term, FactorizedProjectorTerm) or hasattr(term, "phases"):
                 beta = np.asarray(term.beta)
@@ -280,15 +283,28 @@ class PlaneWaveHamiltonian:
             if b.shape[0] != self.npw:
                 raise ValueError("projector rows must match the plane-wave basis")
         nproj = sum(b.shape[1] for b in blocks)
-        beta_host = np.empty((nproj, self.npw), dtype=np.complex128)
         d_host = np.zeros((nproj, nproj), dtype=np.complex128)
         off = 0
         for b, c in zip(blocks, couplings):
             k = b.shape[1]
-            beta_host[off:off + k] = b[perm].T
             d_host[off:off + k, off:off + k] = c
             off += k
-        beta = torch.from_numpy(beta_host).to(self.device)
+        if any(isinstance(b, torch.Tensor) for b in blocks):
+            beta = torch.empty((nproj, self.npw), dtype=torch.complex128, device=self.device)
+            off = 0
+            for b in blocks:
+                k = b.shape[1]
+                beta[off:off + k] = _upload_block(b, self.plan.perm_device, self.npw,
+                                                  self.device, torch.complex128)
+                off += k
+        else:
+            beta_host = np.empty((nproj, self.npw), dtype=np.complex128)
+            off = 0
+            for b in blocks:
+                k = b.shape[1]
+                beta_host[off:off + k] = b[perm].T
+                off += k
+            beta = torch.from_numpy(beta_host).to(self.device)
         # column-major D: torch row j holds column j of D
         coupling = torch.from_numpy(np.ascontiguousarray(d_host.T)).to(self.device)
         return beta, coupling
