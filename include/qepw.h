@@ -229,6 +229,46 @@ int qepw_scale_rows(int precision, const void* src, int64_t lds, const double* s
                     int64_t nrows, int nb, void* stream);
 
 /*
+ * y[:, j] = a[j] x[:, j] + b[j] y[:, j] for j < ncols (a, b: device double,
+ * NULL for 1; y is not read where b[j] == 0).
+ * Replaces: the per-column NumPy updates of the iterative solvers
+ *   (qepy_pw/diagonalization.py:1105-1160 CG, 1225-1310 bpcg,
+ *   1721-1729 RMM-DIIS).
+ */
+int qepw_column_axpby(int precision, const void* x, int64_t ldx, const double* a, void* y,
+                      int64_t ldy, const double* b, int64_t nrows, int ncols, void* stream);
+
+/*
+ * out[b] = sum_g d[g] |v[g, b]|^2  (d, out: device double; FP64 accumulation,
+ * fixed reduction order).
+ * Replaces: column_diagonal_expectations (qepy_pw/_native_fft.pyx:1294-1343).
+ */
+int qepw_column_diagonal_expectations(int precision, const void* v, int64_t ldv,
+                                      const double* d, int64_t nrows, int nb, double* out,
+                                      void* stream);
+
+/*
+ * out[:, j] = K_j(g) r[:, c] with c = cols[j] (or j when cols is NULL),
+ * x = kinetic[g] / (1.5 e[j]), p = 27 + 18x + 12x^2 + 8x^3,
+ * K = -4/(3 e[j]) p / (p + 16 x^4)  (kinetic in Ry, e = <T>_j: device double).
+ * Replaces: rmm_kinetic_precondition (qepy_pw/_native_fft.pyx:1346-1408).
+ */
+int qepw_rmm_precondition(int precision, const void* r, int64_t ldr, const int32_t* cols,
+                          const double* kinetic, const double* expectations, void* out,
+                          int64_t ldo, int64_t nrows, int ncols, void* stream);
+
+/*
+ * out[:, j] = sum_{i < count} coef[j, i] hist_i[:, c] with hist_i = hist + i *
+ * hstride elements, c = cols[j] (or j), coef: device (ncols, count) complex
+ * double pairs, row-major.
+ * Replaces: the per-band DIIS combinations phi @ c and H phi @ c of rmm_diis
+ *   (qepy_pw/diagonalization.py:1655-1662).
+ */
+int qepw_history_combine(int precision, const void* hist, int64_t ldh, int64_t hstride,
+                         int count, const int32_t* cols, const double* coef, void* out,
+                         int64_t ldo, int64_t nrows, int ncols, void* stream);
+
+/*
  * out[g, a*nch + i] = beta[g, i] exp(-i q_g . tau_a): the atom-packed
  * projector matrix recomputed from positions (q: device double (npw, 3)
  * Cartesian k+G, tau: device double (natom, 3) bohr).

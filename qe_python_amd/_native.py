@@ -69,6 +69,14 @@ SIGNATURES = [
                                 c_int64, c_int64, c_int, c_void_p]),
     ("qepw_build_projectors", c_int, [c_int, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_int,
                                       c_void_p, c_int64, c_int64, c_void_p]),
+    ("qepw_column_axpby", c_int, [c_int, c_void_p, c_int64, c_void_p, c_void_p, c_int64,
+                                  c_void_p, c_int64, c_int, c_void_p]),
+    ("qepw_column_diagonal_expectations", c_int, [c_int, c_void_p, c_int64, c_void_p, c_int64,
+                                                  c_int, c_void_p, c_void_p]),
+    ("qepw_rmm_precondition", c_int, [c_int, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_int64, c_int64, c_int, c_void_p]),
+    ("qepw_history_combine", c_int, [c_int, c_void_p, c_int64, c_int64, c_int, c_void_p,
+                                     c_void_p, c_void_p, c_int64, c_int64, c_int, c_void_p]),
     ("qepw_projector_basis", c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int, c_double,
                                      c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int64,
                                      c_void_p]),

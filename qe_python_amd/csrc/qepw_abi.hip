@@ -475,6 +475,53 @@ int qepw_scale_rows(int precision, const void* src, int64_t lds, const double* s
                 "qepw_scale_rows");
 }
 
+int qepw_column_axpby(int precision, const void* x, int64_t ldx, const double* a, void* y,
+                      int64_t ldy, const double* b, int64_t nrows, int ncols, void* stream) {
+  if (nrows < 0 || ncols < 0) return fail(QEPW_ERR_ARG, "qepw_column_axpby: negative size");
+  if (nrows > 0 && ncols > 0 && (!x || !y || ldx < nrows || ldy < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_column_axpby: bad argument");
+  return finish(qepw::column_axpby(precision, x, ldx, a, y, ldy, b, nrows, ncols,
+                                   static_cast<hipStream_t>(stream)),
+                "qepw_column_axpby");
+}
+
+int qepw_column_diagonal_expectations(int precision, const void* v, int64_t ldv,
+                                      const double* d, int64_t nrows, int nb, double* out,
+                                      void* stream) {
+  if (nrows < 0 || nb < 0)
+    return fail(QEPW_ERR_ARG, "qepw_column_diagonal_expectations: negative size");
+  if (nb > 0 && (!v || !d || !out || ldv < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_column_diagonal_expectations: bad argument");
+  return finish(qepw::column_diagonal_expectations(precision, v, ldv, d, nrows, nb, out,
+                                                   static_cast<hipStream_t>(stream)),
+                "qepw_column_diagonal_expectations");
+}
+
+int qepw_rmm_precondition(int precision, const void* r, int64_t ldr, const int32_t* cols,
+                          const double* kinetic, const double* expectations, void* out,
+                          int64_t ldo, int64_t nrows, int ncols, void* stream) {
+  if (nrows < 0 || ncols < 0) return fail(QEPW_ERR_ARG, "qepw_rmm_precondition: negative size");
+  if (nrows > 0 && ncols > 0 &&
+      (!r || !kinetic || !expectations || !out || ldr < nrows || ldo < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_rmm_precondition: bad argument");
+  return finish(qepw::rmm_precondition(precision, r, ldr, cols, kinetic, expectations, out, ldo,
+                                       nrows, ncols, static_cast<hipStream_t>(stream)),
+                "qepw_rmm_precondition");
+}
+
+int qepw_history_combine(int precision, const void* hist, int64_t ldh, int64_t hstride,
+                         int count, const int32_t* cols, const double* coef, void* out,
+                         int64_t ldo, int64_t nrows, int ncols, void* stream) {
+  if (nrows < 0 || ncols < 0 || count < 0)
+    return fail(QEPW_ERR_ARG, "qepw_history_combine: negative size");
+  if (nrows > 0 && ncols > 0 &&
+      (!hist || !coef || !out || count < 1 || ldh < nrows || ldo < nrows || hstride < 0))
+    return fail(QEPW_ERR_ARG, "qepw_history_combine: bad argument");
+  return finish(qepw::history_combine(precision, hist, ldh, hstride, count, cols, coef, out, ldo,
+                                      nrows, ncols, static_cast<hipStream_t>(stream)),
+                "qepw_history_combine");
+}
+
 int qepw_build_projectors(int precision, const void* beta, int64_t ldb, int nch, const double* q,
                           const double* tau, int natom, void* out, int64_t ldo, int64_t nrows,
                           void* stream) {

@@ -63,6 +63,17 @@ int projector_forces(int precision, const void* dO, int64_t lddo, int64_t dstrid
                      const void* DO, int64_t lddO, const double* f, int nb, const int* off,
                      int natom, double* out, hipStream_t stream);
 
+int column_axpby(int precision, const void* x, int64_t ldx, const double* a, void* y,
+                 int64_t ldy, const double* b, int64_t nrows, int ncols, hipStream_t stream);
+int column_diagonal_expectations(int precision, const void* v, int64_t ldv, const double* d,
+                                 int64_t nrows, int nb, double* out, hipStream_t stream);
+int rmm_precondition(int precision, const void* r, int64_t ldr, const int* cols,
+                     const double* kinetic, const double* expect, void* out, int64_t ldo,
+                     int64_t nrows, int ncols, hipStream_t stream);
+int history_combine(int precision, const void* hist, int64_t ldh, int64_t hstride, int count,
+                    const int* cols, const double* coef, void* out, int64_t ldo, int64_t nrows,
+                    int ncols, hipStream_t stream);
+
 int projector_basis(const double* q, int64_t npw, const double* table, int64_t ldt, int nq,
                     double dq, const int* chan_radial, const int* chan_l, const int* chan_m,
                     int nch, void* beta, int64_t ldb, hipStream_t stream);

@@ -351,6 +351,148 @@ int row_real_dot(const void* a, int64_t lda, const void* b, int64_t ldb, int64_t
 
 namespace qepw {
 
+// y[:, j] = a[j] x[:, j] + b[j] y[:, j] (a, b: device double, NULL for 1;
+// y is not read where b[j] == 0): the per-column vector updates of the
+// iterative solvers -- CG's line rotation and conjugation
+// (qepy_pw/diagonalization.py:1105-1160), the bpcg steps and their rollback
+// (1225-1310), the RMM-DIIS step (1721-1729).
+template <typename T>
+__global__ void k_col_axpby(const cplx<T>* __restrict__ x, int64_t ldx,
+                            const double* __restrict__ a, cplx<T>* __restrict__ y, int64_t ldy,
+                            const double* __restrict__ bcoef, int64_t nrows) {
+  const int64_t j = blockIdx.y;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrows) return;
+  const double fa = a ? a[j] : 1.0;
+  const double fb = bcoef ? bcoef[j] : 1.0;
+  const cplx<T> xv = x[j * ldx + g];
+  double re = fa * (double)xv.x, im = fa * (double)xv.y;
+  if (fb != 0.0) {
+    const cplx<T> yv = y[j * ldy + g];
+    re += fb * (double)yv.x;
+    im += fb * (double)yv.y;
+  }
+  y[j * ldy + g] = mk<T>((T)re, (T)im);
+}
+
+// column_diagonal_expectations (_native_fft.pyx:1294-1343):
+// out[b] = sum_g d[g] |v_gb|^2, fixed-order block tree
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_col_diag_expect(const cplx<T>* __restrict__ v,
+                                                                 int64_t ldv,
+                                                                 const double* __restrict__ d,
+                                                                 int64_t nrows,
+                                                                 double* __restrict__ out) {
+  __shared__ double sh[kRedThreads];
+  const int64_t b = blockIdx.x;
+  const cplx<T>* col = v + b * ldv;
+  double acc = 0.0;
+  for (int64_t g = threadIdx.x; g < nrows; g += kRedThreads) {
+    const cplx<T> z = col[g];
+    acc += d[g] * ((double)z.x * (double)z.x + (double)z.y * (double)z.y);
+  }
+  const double s = block_sum<T>(acc, sh);
+  if (threadIdx.x == 0) out[b] = s;
+}
+
+// rmm_kinetic_precondition (_native_fft.pyx:1346-1408): the
+// Kresse-Furthmueller diagonal K(x) = -4/(3 <T>) p(x) / (p(x) + 16 x^4),
+// p(x) = 27 + 18x + 12x^2 + 8x^3, x = T_g / (1.5 <T>), applied to column
+// cols[j] (or j) of r.
+template <typename T>
+__global__ void k_rmm_precondition(const cplx<T>* __restrict__ r, int64_t ldr,
+                                   const int* __restrict__ cols,
+                                   const double* __restrict__ kinetic,
+                                   const double* __restrict__ expect, cplx<T>* __restrict__ out,
+                                   int64_t ldo, int64_t nrows) {
+  const int64_t j = blockIdx.y;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrows) return;
+  const int64_t c = cols ? cols[j] : j;
+  const double e = expect[j];
+  const double x = kinetic[g] / (1.5 * e);
+  const double x2 = x * x;
+  const double num = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x));
+  const double den = num + 16.0 * x2 * x2;
+  const double f = (-4.0 / 3.0) * num / (e * den);
+  const cplx<T> v = r[c * ldr + g];
+  out[j * ldo + g] = mk<T>((T)(f * (double)v.x), (T)(f * (double)v.y));
+}
+
+// out[:, j] = sum_i coef[j, i] hist_i[:, cols[j]] with hist_i = hist + i *
+// hstride and coef (ncols, count) complex row-major: the DIIS combination of
+// each band's own history (qepy_pw/diagonalization.py:1655-1662).
+template <typename T>
+__global__ void k_history_combine(const cplx<T>* __restrict__ hist, int64_t ldh,
+                                  int64_t hstride, int count, const int* __restrict__ cols,
+                                  const double* __restrict__ coef, cplx<T>* __restrict__ out,
+                                  int64_t ldo, int64_t nrows) {
+  const int64_t j = blockIdx.y;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrows) return;
+  const int64_t c = cols ? cols[j] : j;
+  double re = 0.0, im = 0.0;
+  for (int i = 0; i < count; ++i) {
+    const cplx<T> h = hist[i * hstride + c * ldh + g];
+    const double wr = coef[2 * (j * count + i)], wi = coef[2 * (j * count + i) + 1];
+    re += wr * (double)h.x - wi * (double)h.y;
+    im += wr * (double)h.y + wi * (double)h.x;
+  }
+  out[j * ldo + g] = mk<T>((T)re, (T)im);
+}
+
+int column_axpby(int precision, const void* x, int64_t ldx, const double* a, void* y,
+                 int64_t ldy, const double* b, int64_t nrows, int ncols, hipStream_t stream) {
+  if (ncols <= 0 || nrows <= 0) return QEPW_OK;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_col_axpby<double>, rows_grid(nrows, ncols), dim3(256), 0, stream,
+                         static_cast<const double2*>(x), ldx, a, static_cast<double2*>(y), ldy,
+                         b, nrows),
+      hipLaunchKernelGGL(k_col_axpby<float>, rows_grid(nrows, ncols), dim3(256), 0, stream,
+                         static_cast<const float2*>(x), ldx, a, static_cast<float2*>(y), ldy,
+                         b, nrows));
+  return status();
+}
+
+int column_diagonal_expectations(int precision, const void* v, int64_t ldv, const double* d,
+                                 int64_t nrows, int nb, double* out, hipStream_t stream) {
+  if (nb <= 0) return QEPW_OK;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_col_diag_expect<double>, dim3(nb), dim3(kRedThreads), 0, stream,
+                         static_cast<const double2*>(v), ldv, d, nrows, out),
+      hipLaunchKernelGGL(k_col_diag_expect<float>, dim3(nb), dim3(kRedThreads), 0, stream,
+                         static_cast<const float2*>(v), ldv, d, nrows, out));
+  return status();
+}
+
+int rmm_precondition(int precision, const void* r, int64_t ldr, const int* cols,
+                     const double* kinetic, const double* expect, void* out, int64_t ldo,
+                     int64_t nrows, int ncols, hipStream_t stream) {
+  if (ncols <= 0 || nrows <= 0) return QEPW_OK;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_rmm_precondition<double>, rows_grid(nrows, ncols), dim3(256), 0,
+                         stream, static_cast<const double2*>(r), ldr, cols, kinetic, expect,
+                         static_cast<double2*>(out), ldo, nrows),
+      hipLaunchKernelGGL(k_rmm_precondition<float>, rows_grid(nrows, ncols), dim3(256), 0,
+                         stream, static_cast<const float2*>(r), ldr, cols, kinetic, expect,
+                         static_cast<float2*>(out), ldo, nrows));
+  return status();
+}
+
+int history_combine(int precision, const void* hist, int64_t ldh, int64_t hstride, int count,
+                    const int* cols, const double* coef, void* out, int64_t ldo, int64_t nrows,
+                    int ncols, hipStream_t stream) {
+  if (ncols <= 0 || nrows <= 0) return QEPW_OK;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_history_combine<double>, rows_grid(nrows, ncols), dim3(256), 0,
+                         stream, static_cast<const double2*>(hist), ldh, hstride, count, cols,
+                         coef, static_cast<double2*>(out), ldo, nrows),
+      hipLaunchKernelGGL(k_history_combine<float>, rows_grid(nrows, ncols), dim3(256), 0,
+                         stream, static_cast<const float2*>(hist), ldh, hstride, count, cols,
+                         coef, static_cast<float2*>(out), ldo, nrows));
+  return status();
+}
+
 int scale_rows(int precision, const void* src, int64_t lds, const double* s, int64_t sstride,
                const double* cfac, void* dst, int64_t ldd, int64_t nrows, int nb,
                hipStream_t stream) {

@@ -229,6 +229,60 @@ def accumulate_projector_diagonal(coupled: torch.Tensor, beta: torch.Tensor, nro
     nat.check(rc, "qepw_accumulate_projector_diagonal")
 
 
+def column_axpby(x: torch.Tensor, a: torch.Tensor | None, y: torch.Tensor,
+                 b: torch.Tensor | None, nrows: int) -> None:
+    """y[j] = a[j] x[j] + b[j] y[j] over the rows of (ncols, nrows) blocks (a, b:
+    device float64 per row, None for 1)."""
+    ncols = int(y.shape[0])
+    if x.shape[0] < ncols or (a is not None and a.numel() < ncols) or (
+            b is not None and b.numel() < ncols):
+        raise ValueError("column_axpby operands are shorter than the updated block")
+    rc = nat.load().qepw_column_axpby(precision_code(y.dtype), x.data_ptr(), ld_of(x),
+                                      None if a is None else a.data_ptr(), y.data_ptr(), ld_of(y),
+                                      None if b is None else b.data_ptr(), nrows, ncols,
+                                      stream_handle(y.device))
+    nat.check(rc, "qepw_column_axpby")
+
+
+def column_diagonal_expectations(v: torch.Tensor, d: torch.Tensor, nrows: int) -> torch.Tensor:
+    nb = int(v.shape[0])
+    out = torch.empty(nb, dtype=torch.float64, device=v.device)
+    rc = nat.load().qepw_column_diagonal_expectations(precision_code(v.dtype), v.data_ptr(),
+                                                      ld_of(v), d.data_ptr(), nrows, nb,
+                                                      out.data_ptr(), stream_handle(v.device))
+    nat.check(rc, "qepw_column_diagonal_expectations")
+    return out
+
+
+def rmm_precondition(r: torch.Tensor, cols: torch.Tensor | None, kinetic: torch.Tensor,
+                     expectations: torch.Tensor, out: torch.Tensor, nrows: int) -> None:
+    ncols = int(out.shape[0])
+    if expectations.numel() < ncols or (cols is not None and cols.numel() < ncols):
+        raise ValueError("rmm_precondition metadata is shorter than the output block")
+    rc = nat.load().qepw_rmm_precondition(precision_code(r.dtype), r.data_ptr(), ld_of(r),
+                                          None if cols is None else cols.data_ptr(),
+                                          kinetic.data_ptr(), expectations.data_ptr(),
+                                          out.data_ptr(), ld_of(out), nrows, ncols,
+                                          stream_handle(r.device))
+    nat.check(rc, "qepw_rmm_precondition")
+
+
+def history_combine(hist: torch.Tensor, count: int, cols: torch.Tensor | None,
+                    coef: torch.Tensor, out: torch.Tensor, nrows: int) -> None:
+    """out[j] = sum_i coef[j, i] hist[i, cols[j]] for a (depth, nb, nrows)
+    history and (ncols, count) complex128 coefficients."""
+    ncols = int(out.shape[0])
+    if count > hist.shape[0] or tuple(coef.shape) != (ncols, count) or (
+            cols is not None and cols.numel() < ncols):
+        raise ValueError("history_combine operands do not match")
+    rc = nat.load().qepw_history_combine(precision_code(hist.dtype), hist.data_ptr(),
+                                         hist.stride(1), hist.stride(0), count,
+                                         None if cols is None else cols.data_ptr(),
+                                         coef.data_ptr(), out.data_ptr(), ld_of(out), nrows,
+                                         ncols, stream_handle(out.device))
+    nat.check(rc, "qepw_history_combine")
+
+
 class DeviceFFTPlan:
     """Device plan of the sphere-pruned V_loc FFT for one k-point basis
     (device counterpart of the serial ``LocalPotentialWorkspace`` tables,

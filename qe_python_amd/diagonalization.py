@@ -1091,18 +1091,21 @@ def _result_vectors(X, perm_dev, npw, device_output=False):
 
 
 def conjugate_gradient(*args, **kwargs):
-    """Band-by-band CG (reference ``diagonalization.py:993-1186``): not part
-    of the device path yet."""
-    raise NotImplementedError("conjugate_gradient is not implemented on the device path")
+    """Band-by-band CG (reference ``diagonalization.py:993-1186``) on the
+    device ``h_psi``: ``qe_python_amd.iterative.conjugate_gradient``."""
+    from .iterative import conjugate_gradient as solver
+    return solver(*args, **kwargs)
 
 
 def parallel_orbital(*args, **kwargs):
-    """ParO (reference ``diagonalization.py:1328-1467``): not part of the
-    device path yet."""
-    raise NotImplementedError("parallel_orbital is not implemented on the device path")
+    """ParO (reference ``diagonalization.py:1328-1467``) on the device
+    ``h_psi``: ``qe_python_amd.iterative.parallel_orbital``."""
+    from .iterative import parallel_orbital as solver
+    return solver(*args, **kwargs)
 
 
 def rmm_diis(*args, **kwargs):
-    """RMM-DIIS (reference ``diagonalization.py:1528-1756``): not part of the
-    device path yet."""
-    raise NotImplementedError("rmm_diis is not implemented on the device path")
+    """RMM-DIIS (reference ``diagonalization.py:1528-1756``) on the device
+    ``h_psi``: ``qe_python_amd.iterative.rmm_diis``."""
+    from .iterative import rmm_diis as solver
+    return solver(*args, **kwargs)
