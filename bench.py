@@ -396,6 +396,62 @@ def projector_cpu_baseline(system, inputs, packed, device_ms):
             "max_rel_error": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))}
 
 
+def iterative_leg(system, ham, trial):
+    """The reference's other solvers on the device h_psi, medium workload, 128
+    bands from the same trial vectors as the Davidson solves: one RMM-DIIS
+    call (four DIIS steps, as each SCF iteration makes it), ParO to Ritz-value
+    changes below 1e-6 Ha, and band-by-band CG with at most 8 iterations per
+    band.  Wall time of the second of two calls, synchronized."""
+    import numpy as np
+    import torch
+
+    from qe_python_amd.diagonalization import conjugate_gradient, parallel_orbital, rmm_diis
+    n = 128
+    out = {"workload": "medium: 128 bands from the trial block"}
+    cases = (("rmm_diis", rmm_diis, dict(tolerance=1e-8)),
+             ("parallel_orbital", parallel_orbital, dict(tolerance=1e-6, max_iterations=20)),
+             ("conjugate_gradient", conjugate_gradient, dict(tolerance=1e-6, max_iterations=8)))
+    results = {}
+    for name, fn, kw in cases:
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            res = fn(ham.apply, ham.diagonal, n, initial_vectors=trial[:, :n],
+                     operator_into=ham.apply_into, **kw)
+            torch.cuda.synchronize()
+            ms = 1e3 * (time.perf_counter() - t)
+        results[name] = res
+        out[name] = {"ms_per_call": ms, "hamiltonian_applications": res.hamiltonian_applications,
+                     "converged": bool(res.converged), "options": kw,
+                     "max_residual_norm": float(np.max(res.residual_norms))}
+    return out, results["rmm_diis"]
+
+
+def iterative_cpu_baseline(system, trial, device_result, device_ms):
+    """One oracle RMM-DIIS call on the host (same bands, same Hamiltonian
+    built by the oracle), and the device call's eigenvalues checked
+    against it."""
+    from types import SimpleNamespace
+
+    import numpy as np
+
+    from oracle.hpsi import OracleHamiltonian
+    from oracle.iterative import rmm_diis as oracle_rmm
+    beta, coup, diag = system.packed_term()
+    term = SimpleNamespace(beta=beta, coupling=coup, diagonal=diag)
+    ref = OracleHamiltonian(system.basis.indices, system.basis.kinetic, system.shape,
+                            system.real_potential, (term,),
+                            potential_average=float(np.real(system.potential_g.flat[0])))
+    t = time.perf_counter()
+    res = oracle_rmm(ref.apply, ref.diagonal, 128, initial_vectors=trial[:, :128], tolerance=1e-8)
+    cpu_ms = 1e3 * (time.perf_counter() - t)
+    return {"cpu_ms_oracle": cpu_ms, "gpu_over_cpu": cpu_ms / device_ms,
+            "max_eigenvalue_difference_ha": float(np.max(np.abs(
+                res.eigenvalues - device_result.eigenvalues))),
+            "same_applications": res.hamiltonian_applications ==
+            device_result.hamiltonian_applications}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -527,6 +583,8 @@ def main():
     line["density"] = density
     projectors, p_inputs, p_packed = projector_leg(system, device)
     line["projectors"] = projectors
+    iterative, rmm_result = iterative_leg(system, ham, trial)
+    line["iterative"] = iterative
     if not args.no_cpu:
         happ = head["hamiltonian_applications"][-1]
         line["cpu_baseline"] = cpu_baseline(system, kwargs, happ, args.cpu_budget)
@@ -534,6 +592,8 @@ def main():
                                                     density["ms_per_call"]))
         line["projectors"].update(projector_cpu_baseline(system, p_inputs, p_packed,
                                                          projectors["ms_per_call"]))
+        line["iterative"]["rmm_diis"].update(iterative_cpu_baseline(
+            system, system.trial, rmm_result, iterative["rmm_diis"]["ms_per_call"]))
         if head["value"] > 0:
             line["config"]["gpu_over_cpu"] = head["value"] / line["cpu_baseline"]["value"]
     print(json.dumps(line))
