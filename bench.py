@@ -452,6 +452,52 @@ def iterative_cpu_baseline(system, trial, device_result, device_ms):
             device_result.hamiltonian_applications}
 
 
+def wavefunction_io_leg(system, device, with_cpu):
+    """The SCF's wavefunction working file (WavefunctionBuffer,
+    pw/buffers.py:109-232) with device eigenvectors: one medium record (128
+    bands, npw 23847, 48.8 MB) written from and read back to the device, in a
+    temporary directory; with ``with_cpu``, the host path (the reference's
+    own transpose-and-pad write and read of the same matrix) beside it.
+    Wall times of the third of three calls each, synchronized."""
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from qe_python_amd.wavefunctions import WavefunctionBuffer
+    nb = 128
+    coeff = np.ascontiguousarray(system.trial[:, :nb].T)
+    dev = torch.from_numpy(coeff).to(device).T          # davidson's device_output layout
+    out = {"workload": f"medium: one record of {nb} bands x npw {system.npw}"}
+    with tempfile.TemporaryDirectory() as tmp:
+        buf = WavefunctionBuffer(tmp, "bench", [system.npw], nb)
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            buf.write(0, dev)
+            w_ms = 1e3 * (time.perf_counter() - t)
+            t = time.perf_counter()
+            back = buf.read_device(0, device)
+            torch.cuda.synchronize()
+            r_ms = 1e3 * (time.perf_counter() - t)
+        ok = bool(torch.equal(back, dev))
+        mb = buf.record_bytes / 1e6
+        out.update({"record_MB": mb, "device_write_ms": w_ms, "device_read_ms": r_ms,
+                    "write_GBps": mb / w_ms, "read_GBps": mb / r_ms, "round_trip_exact": ok})
+        if with_cpu:
+            host = np.asfortranarray(system.trial[:, :nb])
+            for _ in range(3):
+                t = time.perf_counter()
+                buf.write(0, host)
+                hw = 1e3 * (time.perf_counter() - t)
+                t = time.perf_counter()
+                hb = buf[0]
+                hr = 1e3 * (time.perf_counter() - t)
+            out.update({"host_write_ms": hw, "host_read_ms": hr,
+                        "host_round_trip_exact": bool(np.array_equal(hb, host))})
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -585,6 +631,7 @@ def main():
     line["projectors"] = projectors
     iterative, rmm_result = iterative_leg(system, ham, trial)
     line["iterative"] = iterative
+    line["wavefunction_io"] = wavefunction_io_leg(system, device, not args.no_cpu)
     if not args.no_cpu:
         happ = head["hamiltonian_applications"][-1]
         line["cpu_baseline"] = cpu_baseline(system, kwargs, happ, args.cpu_budget)

@@ -290,7 +290,7 @@ class DeviceFFTPlan:
 
     def __init__(self, miller: np.ndarray, shape, device: torch.device):
         lib = nat.load()
-        self.device = device
+        self.device = device = require_device(device)
         miller = np.ascontiguousarray(miller, dtype=np.int32)
         if miller.ndim != 2 or miller.shape[1] != 3:
             raise ValueError("indices must have shape (nplane, 3)")
