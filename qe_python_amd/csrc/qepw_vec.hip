@@ -22,41 +22,60 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return sh[0];
 }
 
+// Fixed-order tree sum over a THREADS-wide workgroup.
+template <int THREADS>
+__device__ __forceinline__ double tree_sum(double v, double* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int s = THREADS / 2; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] += sh[tid + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// Blocks of at most this many columns reduce each column with a 1024-wide
+// workgroup instead of 256 (a single column of npw ~ 24k is otherwise a
+// ~30 us serial loop: the band-by-band CG's inner products).
+constexpr int kWideColumns = 64;
+
 // column_squared_norms (_native_fft.pyx:1153-1191): out[b] = sum_g |v_gb|^2
-template <typename T>
-__global__ __launch_bounds__(kRedThreads) void k_col_sqnorm(const cplx<T>* __restrict__ v,
-                                                            int64_t ldv, int64_t nrows,
-                                                            double* __restrict__ out) {
-  __shared__ double sh[kRedThreads];
+template <typename T, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_col_sqnorm(const cplx<T>* __restrict__ v,
+                                                        int64_t ldv, int64_t nrows,
+                                                        double* __restrict__ out) {
+  __shared__ double sh[THREADS];
   const int64_t b = blockIdx.x;
   const cplx<T>* col = v + b * ldv;
   double acc = 0.0;
-  for (int64_t g = threadIdx.x; g < nrows; g += kRedThreads) {
+  for (int64_t g = threadIdx.x; g < nrows; g += THREADS) {
     const cplx<T> z = col[g];
     acc += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
   }
-  const double s = block_sum<T>(acc, sh);
+  const double s = tree_sum<THREADS>(acc, sh);
   if (threadIdx.x == 0) out[b] = s;
 }
 
 // column_inner_products (_native_fft.pyx:1241-1291): out[b] = <l_b | r_b>
-template <typename T>
-__global__ __launch_bounds__(kRedThreads) void k_col_inner(const cplx<T>* __restrict__ l,
-                                                           int64_t ldl,
-                                                           const cplx<T>* __restrict__ r,
-                                                           int64_t ldr, int64_t nrows,
-                                                           double* __restrict__ out) {
-  __shared__ double sh[kRedThreads];
+template <typename T, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_col_inner(const cplx<T>* __restrict__ l,
+                                                       int64_t ldl,
+                                                       const cplx<T>* __restrict__ r,
+                                                       int64_t ldr, int64_t nrows,
+                                                       double* __restrict__ out) {
+  __shared__ double sh[THREADS];
   const int64_t b = blockIdx.x;
   double ar = 0.0, ai = 0.0;
-  for (int64_t g = threadIdx.x; g < nrows; g += kRedThreads) {
+  for (int64_t g = threadIdx.x; g < nrows; g += THREADS) {
     const cplx<T> x = l[b * ldl + g];
     const cplx<T> y = r[b * ldr + g];
     ar += (double)x.x * y.x + (double)x.y * y.y;
     ai += (double)x.x * y.y - (double)x.y * y.x;
   }
-  const double sr = block_sum<T>(ar, sh);
-  const double si = block_sum<T>(ai, sh);
+  const double sr = tree_sum<THREADS>(ar, sh);
+  __syncthreads();  // every thread has read sh[0] before it is reused
+  const double si = tree_sum<THREADS>(ai, sh);
   if (threadIdx.x == 0) { out[2 * b] = sr; out[2 * b + 1] = si; }
 }
 
@@ -233,10 +252,18 @@ int column_squared_norms(int precision, const void* v, int64_t ldv, int64_t nrow
                          double* out, hipStream_t stream) {
   if (nb <= 0) return QEPW_OK;
   QEPW_DISPATCH(precision,
-      hipLaunchKernelGGL(k_col_sqnorm<double>, dim3(nb), dim3(kRedThreads), 0, stream,
-                         static_cast<const double2*>(v), ldv, nrows, out),
-      hipLaunchKernelGGL(k_col_sqnorm<float>, dim3(nb), dim3(kRedThreads), 0, stream,
-                         static_cast<const float2*>(v), ldv, nrows, out));
+      if (nb <= kWideColumns)
+        hipLaunchKernelGGL((k_col_sqnorm<double, 1024>), dim3(nb), dim3(1024), 0, stream,
+                           static_cast<const double2*>(v), ldv, nrows, out);
+      else
+        hipLaunchKernelGGL((k_col_sqnorm<double, kRedThreads>), dim3(nb), dim3(kRedThreads), 0,
+                           stream, static_cast<const double2*>(v), ldv, nrows, out),
+      if (nb <= kWideColumns)
+        hipLaunchKernelGGL((k_col_sqnorm<float, 1024>), dim3(nb), dim3(1024), 0, stream,
+                           static_cast<const float2*>(v), ldv, nrows, out);
+      else
+        hipLaunchKernelGGL((k_col_sqnorm<float, kRedThreads>), dim3(nb), dim3(kRedThreads), 0,
+                           stream, static_cast<const float2*>(v), ldv, nrows, out));
   return status();
 }
 
@@ -245,12 +272,22 @@ int column_inner_products(int precision, const void* l, int64_t ldl, const void*
                           hipStream_t stream) {
   if (nb <= 0) return QEPW_OK;
   QEPW_DISPATCH(precision,
-      hipLaunchKernelGGL(k_col_inner<double>, dim3(nb), dim3(kRedThreads), 0, stream,
-                         static_cast<const double2*>(l), ldl,
-                         static_cast<const double2*>(r), ldr, nrows, out),
-      hipLaunchKernelGGL(k_col_inner<float>, dim3(nb), dim3(kRedThreads), 0, stream,
-                         static_cast<const float2*>(l), ldl,
-                         static_cast<const float2*>(r), ldr, nrows, out));
+      if (nb <= kWideColumns)
+        hipLaunchKernelGGL((k_col_inner<double, 1024>), dim3(nb), dim3(1024), 0, stream,
+                           static_cast<const double2*>(l), ldl,
+                           static_cast<const double2*>(r), ldr, nrows, out);
+      else
+        hipLaunchKernelGGL((k_col_inner<double, kRedThreads>), dim3(nb), dim3(kRedThreads), 0,
+                           stream, static_cast<const double2*>(l), ldl,
+                           static_cast<const double2*>(r), ldr, nrows, out),
+      if (nb <= kWideColumns)
+        hipLaunchKernelGGL((k_col_inner<float, 1024>), dim3(nb), dim3(1024), 0, stream,
+                           static_cast<const float2*>(l), ldl,
+                           static_cast<const float2*>(r), ldr, nrows, out);
+      else
+        hipLaunchKernelGGL((k_col_inner<float, kRedThreads>), dim3(nb), dim3(kRedThreads), 0,
+                           stream, static_cast<const float2*>(l), ldl,
+                           static_cast<const float2*>(r), ldr, nrows, out));
   return status();
 }
 

@@ -262,11 +262,12 @@ def conjugate_gradient(operator, diagonal, number_of_roots, *, initial_vectors,
             s.axpby(ppsi, -e_p, g, None)
             if band:
                 s.project_out(lower, g)
+            pg = s.scale(g, s=p)
             if it > 1:
-                (cross,) = s.dots((g, pg_prev))
-                cross = float(np.real(cross[0]))
-            pg_prev = s.scale(g, s=p)
-            gg = float(np.real(s.dots((g, pg_prev))[0][0]))
+                cross, gg = (float(np.real(x[0])) for x in s.dots((g, pg_prev), (g, pg)))
+            else:
+                gg = float(np.real(s.dots((g, pg))[0][0]))
+            pg_prev = pg
             if not np.isfinite(gg) or gg <= 1.0e-30:
                 break
             if it == 1:
