@@ -269,6 +269,17 @@ int qepw_history_combine(int precision, const void* hist, int64_t ldh, int64_t h
                          int64_t ldo, int64_t nrows, int ncols, void* stream);
 
 /*
+ * out[d, a*nch + i] = beta[r, i] phases[r, a] with r = perm[d] (perm: device
+ * int32, NULL for the identity): the atom blocks of a factorized projector
+ * term packed on the device, optionally gathered into a plan's device order.
+ * Replaces: the per-atom phase products of FactorizedProjectorTerm applied in
+ *   PlaneWaveHamiltonian._apply (qepy_pw/diagonalization.py:21-28, 293-309).
+ */
+int qepw_phase_blocks(int precision, const void* beta, int64_t ldb, int nch,
+                      const void* phases, int64_t ldp, int natom, const int32_t* perm, void* out,
+                      int64_t ldo, int64_t nrows, void* stream);
+
+/*
  * out[g, a*nch + i] = beta[g, i] exp(-i q_g . tau_a): the atom-packed
  * projector matrix recomputed from positions (q: device double (npw, 3)
  * Cartesian k+G, tau: device double (natom, 3) bohr).

@@ -77,6 +77,8 @@ SIGNATURES = [
                                       c_void_p, c_int64, c_int64, c_int, c_void_p]),
     ("qepw_history_combine", c_int, [c_int, c_void_p, c_int64, c_int64, c_int, c_void_p,
                                      c_void_p, c_void_p, c_int64, c_int64, c_int, c_void_p]),
+    ("qepw_phase_blocks", c_int, [c_int, c_void_p, c_int64, c_int, c_void_p, c_int64, c_int,
+                                  c_void_p, c_void_p, c_int64, c_int64, c_void_p]),
     ("qepw_projector_basis", c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int, c_double,
                                      c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int64,
                                      c_void_p]),

@@ -546,6 +546,19 @@ int qepw_projector_basis(const double* q, int64_t npw, const double* table, int6
                 "qepw_projector_basis");
 }
 
+int qepw_phase_blocks(int precision, const void* beta, int64_t ldb, int nch,
+                      const void* phases, int64_t ldp, int natom, const int32_t* perm, void* out,
+                      int64_t ldo, int64_t nrows, void* stream) {
+  if (nch < 0 || natom < 0 || nrows < 0)
+    return fail(QEPW_ERR_ARG, "qepw_phase_blocks: negative size");
+  if (natom > 0 && nch > 0 && nrows > 0 &&
+      (!beta || !phases || !out || ldb < nrows || ldp < nrows || ldo < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_phase_blocks: bad argument");
+  return finish(qepw::phase_blocks(precision, beta, ldb, nch, phases, ldp, natom, perm, out, ldo,
+                                   nrows, static_cast<hipStream_t>(stream)),
+                "qepw_phase_blocks");
+}
+
 int qepw_projector_forces(int precision, const void* dO, int64_t lddo, int64_t dstride,
                           const void* DO, int64_t lddO, const double* f, int nb,
                           const int32_t* off, int natom, double* out, void* stream) {

@@ -59,6 +59,9 @@ int scale_rows(int precision, const void* src, int64_t lds, const double* s, int
 int build_projectors(int precision, const void* beta, int64_t ldb, int nch, const double* q,
                      const double* tau, int natom, void* out, int64_t ldo, int64_t nrows,
                      hipStream_t stream);
+int phase_blocks(int precision, const void* beta, int64_t ldb, int nch, const void* phases,
+                 int64_t ldp, int natom, const int* perm, void* out, int64_t ldo, int64_t nrows,
+                 hipStream_t stream);
 int projector_forces(int precision, const void* dO, int64_t lddo, int64_t dstride,
                      const void* DO, int64_t lddO, const double* f, int nb, const int* off,
                      int natom, double* out, hipStream_t stream);

@@ -211,6 +211,28 @@ __global__ void k_build_projectors(const cplx<T>* __restrict__ beta, int64_t ldb
   }
 }
 
+// out[d, a*nch + i] = beta[r, i] phases[r, a] with r = perm[d] (or d): the
+// atom blocks of a factorized projector term (reference
+// qepy_pw/diagonalization.py:21-28, FactorizedProjectorTerm) packed on the
+// device, optionally gathered into a plan's device row order.
+template <typename T>
+__global__ void k_phase_blocks(const cplx<T>* __restrict__ beta, int64_t ldb, int nch,
+                               const cplx<T>* __restrict__ phases, int64_t ldp, int natom,
+                               const int* __restrict__ perm, cplx<T>* __restrict__ out,
+                               int64_t ldo, int64_t nrows) {
+  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y;
+  if (d >= nrows || a >= natom) return;
+  const int64_t r = perm ? perm[d] : d;
+  const cplx<T> ph = phases[(int64_t)a * ldp + r];
+  for (int i = 0; i < nch; ++i) {
+    const cplx<T> bv = beta[(int64_t)i * ldb + r];
+    const double re = (double)bv.x * ph.x - (double)bv.y * ph.y;
+    const double im = (double)bv.x * ph.y + (double)bv.y * ph.x;
+    out[(int64_t)(a * nch + i) * ldo + d] = mk<T>((T)re, (T)im);
+  }
+}
+
 // out[a*3 + c] = 2 sum_n f_n sum_{i in atom a} Re(conj(dO_c[i, n]) DO[i, n]),
 // the force-branch contraction of pw/scf.py:1591-1599 (dE/dtau = -F).
 // dO: three (P, nb) blocks with column stride lddo and block stride
@@ -645,6 +667,23 @@ int build_projectors(int precision, const void* beta, int64_t ldb, int nch, cons
                          static_cast<double2*>(out), ldo, nrows),
       hipLaunchKernelGGL(k_build_projectors<float>, rows_grid(nrows, natom), dim3(256), 0,
                          stream, static_cast<const float2*>(beta), ldb, nch, q, tau, natom,
+                         static_cast<float2*>(out), ldo, nrows));
+  return status();
+}
+
+int phase_blocks(int precision, const void* beta, int64_t ldb, int nch, const void* phases,
+                 int64_t ldp, int natom, const int* perm, void* out, int64_t ldo, int64_t nrows,
+                 hipStream_t stream) {
+  if (natom <= 0 || nrows <= 0 || nch <= 0) return QEPW_OK;
+  if (natom > 65535) return QEPW_ERR_UNSUPPORTED;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_phase_blocks<double>, rows_grid(nrows, natom), dim3(256), 0, stream,
+                         static_cast<const double2*>(beta), ldb, nch,
+                         static_cast<const double2*>(phases), ldp, natom, perm,
+                         static_cast<double2*>(out), ldo, nrows),
+      hipLaunchKernelGGL(k_phase_blocks<float>, rows_grid(nrows, natom), dim3(256), 0, stream,
+                         static_cast<const float2*>(beta), ldb, nch,
+                         static_cast<const float2*>(phases), ldp, natom, perm,
                          static_cast<float2*>(out), ldo, nrows));
   return status();
 }

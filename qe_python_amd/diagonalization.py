@@ -267,6 +267,13 @@ class PlaneWaveHamiltonian:
                               else np.asarray(term.beta))
                 couplings.append(np.asarray(term.coupling))
             elif isinstance(term, FactorizedProjectorTerm) or hasattr(term, "phases"):
+                if isinstance(term.beta, torch.Tensor):
+                    # device-built (qe_python_amd.projectors): the atom blocks
+                    # are packed on the device, in device row order
+                    blocks.append(self._device_phase_blocks(term))
+                    couplings.append(np.kron(np.eye(int(term.phases.shape[1])),
+                                             np.asarray(term.coupling)))
+                    continue
                 beta = np.asarray(term.beta)
                 for atom in range(term.phases.shape[1]):
                     blocks.append(beta * np.asarray(term.phases)[:, atom, None])
@@ -289,13 +296,16 @@ class PlaneWaveHamiltonian:
             k = b.shape[1]
             d_host[off:off + k, off:off + k] = c
             off += k
-        if any(isinstance(b, torch.Tensor) for b in blocks):
+        if any(isinstance(b, (torch.Tensor, _DeviceOrderBlock)) for b in blocks):
             beta = torch.empty((nproj, self.npw), dtype=torch.complex128, device=self.device)
             off = 0
             for b in blocks:
                 k = b.shape[1]
-                beta[off:off + k] = _upload_block(b, self.plan.perm_device, self.npw,
-                                                  self.device, torch.complex128)
+                if isinstance(b, _DeviceOrderBlock):
+                    beta[off:off + k] = b.rows
+                else:
+                    beta[off:off + k] = _upload_block(b, self.plan.perm_device, self.npw,
+                                                      self.device, torch.complex128)
                 off += k
         else:
             beta_host = np.empty((nproj, self.npw), dtype=np.complex128)
@@ -308,6 +318,25 @@ class PlaneWaveHamiltonian:
         # column-major D: torch row j holds column j of D
         coupling = torch.from_numpy(np.ascontiguousarray(d_host.T)).to(self.device)
         return beta, coupling
+
+    def _device_phase_blocks(self, term) -> "_DeviceOrderBlock":
+        """(natom*nch, npw) device-order atom blocks beta * phases[:, a] of a
+        device-built factorized term (``qepw_phase_blocks``)."""
+        beta = term.beta.to(self.device, torch.complex128)
+        phases = term.phases.to(self.device, torch.complex128)
+        if beta.shape[0] != self.npw or phases.shape[0] != self.npw:
+            raise ValueError("projector rows must match the plane-wave basis")
+        nch, natom = int(beta.shape[1]), int(phases.shape[1])
+        b = beta.T.resolve_conj().contiguous()
+        ph = phases.T.resolve_conj().contiguous()
+        out = torch.empty((natom * nch, self.npw), dtype=torch.complex128, device=self.device)
+        perm = self.plan.perm_device
+        rc = nat.load().qepw_phase_blocks(
+            nat.C128, b.data_ptr(), dv.ld_of(b), nch, ph.data_ptr(), dv.ld_of(ph), natom,
+            None if perm is None else perm.data_ptr(), out.data_ptr(), dv.ld_of(out), self.npw,
+            dv.stream_handle(self.device))
+        nat.check(rc, "qepw_phase_blocks")
+        return _DeviceOrderBlock(out)
 
     def _resources(self, dtype: torch.dtype) -> _PrecisionResources:
         res = self._res.get(dtype)
@@ -343,12 +372,15 @@ class PlaneWaveHamiltonian:
                 beta, coupling, mult = term.beta, term.coupling, term.phases.shape[1]
             else:
                 (beta, coupling), mult = term, 1
-            beta = np.asarray(beta)
             if beta.shape[1] == 0:
                 continue
             if acc is None:
                 acc = torch.zeros(self.npw, dtype=torch.float64, device=self.device)
-            b = torch.from_numpy(np.ascontiguousarray(beta.T, dtype=np.complex128)).to(self.device)
+            if isinstance(beta, torch.Tensor):
+                b = beta.T.to(self.device, torch.complex128).resolve_conj().contiguous()
+            else:
+                b = torch.from_numpy(np.ascontiguousarray(np.asarray(beta).T,
+                                                          dtype=np.complex128)).to(self.device)
             d = torch.from_numpy(np.ascontiguousarray(np.asarray(coupling, np.complex128).T)).to(self.device)
             coupled = torch.empty_like(b)
             dv.gemm(nat.OP_N, self.npw, b.shape[0], b.shape[0], 1.0, b, d, 0.0, coupled,
@@ -448,6 +480,15 @@ class PlaneWaveHamiltonian:
             target[...] = result
             return out
         return result[:, 0] if was_vector else result
+
+
+class _DeviceOrderBlock:
+    """Projector columns already on the device in device row order, as a
+    (P, npw) tensor; ``shape`` reads like the (npw, P) host blocks'."""
+
+    def __init__(self, rows: torch.Tensor):
+        self.rows = rows
+        self.shape = (int(rows.shape[1]), int(rows.shape[0]))
 
 
 def _upload_block(vectors, perm, npw, device, dtype):

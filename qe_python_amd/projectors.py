@@ -23,7 +23,7 @@ import torch
 
 from . import _native as nat
 from . import device as dv
-from .diagonalization import PackedProjectorTerm
+from .diagonalization import FactorizedProjectorTerm, PackedProjectorTerm
 
 
 def _device_rows(array, device, dtype=torch.float64) -> torch.Tensor:
@@ -116,3 +116,31 @@ def packed_projector_term(beta_species: torch.Tensor, couplings, positions, gk,
         dv.gemm(nat.OP_N, npw, nch, nch, 1.0, src, dsum, 0.0, coupled, ws)
         dv.accumulate_projector_diagonal(coupled, src, npw, 1.0, diag)
     return PackedProjectorTerm(packed.T, coupling, diag.cpu().numpy())
+
+
+def factorized_projector_term(beta_species: torch.Tensor, coupling, positions, gk,
+                              device=None) -> FactorizedProjectorTerm:
+    """The reference's ``FactorizedProjectorTerm`` (species beta, atom phase
+    columns, one shared coupling; ``pw/scf.py:692-707``, used when a species
+    has more than 64 packed channels) with device tensors: beta (npw, nch) as
+    given, phases[g, a] = exp(-i (k+G)_g . tau_a) computed on the device
+    (``qepw_build_projectors`` of a unit column), rows in the order of
+    ``gk``.  ``PlaneWaveHamiltonian`` packs its atom blocks on the device
+    (``qepw_phase_blocks``)."""
+    dev = dv.require_device(device)
+    bs = beta_species.to(dev, torch.complex128)
+    npw = int(bs.shape[0])
+    coupling = np.asarray(coupling, dtype=np.complex128)
+    if coupling.shape != (bs.shape[1], bs.shape[1]):
+        raise ValueError("coupling must be (nch, nch)")
+    tau = _device_rows(positions, dev)
+    natom = int(tau.shape[0])
+    q = _device_rows(gk, dev)
+    ones = torch.ones((1, npw), dtype=torch.complex128, device=dev)
+    phases = torch.empty((natom, npw), dtype=torch.complex128, device=dev)
+    if npw and natom:
+        rc = nat.load().qepw_build_projectors(
+            nat.C128, ones.data_ptr(), npw, 1, q.data_ptr(), tau.data_ptr(), natom,
+            phases.data_ptr(), npw, npw, dv.stream_handle(dev))
+        nat.check(rc, "qepw_build_projectors")
+    return FactorizedProjectorTerm(bs, phases.T, coupling)

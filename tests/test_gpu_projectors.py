@@ -119,3 +119,34 @@ def test_projector_basis_errors():
     beta = projector_basis(gk, ok, dq, [(0, 0, 0)])
     with pytest.raises(ValueError, match="coupling"):
         packed_projector_term(beta, [np.eye(2)], np.zeros((1, 3)), gk)
+
+
+def test_factorized_term_on_the_device():
+    import torch
+
+    from qe_python_amd.diagonalization import FactorizedProjectorTerm, PlaneWaveHamiltonian
+    from qe_python_amd.projectors import factorized_projector_term, packed_projector_term
+    s = make_system("small")
+    pr = s.projectors
+    gk = s.basis.vectors
+    d = pr.couplings[0]                      # one species: one shared coupling
+    beta_s = torch.from_numpy(np.ascontiguousarray(pr.beta_species))
+    term_d = factorized_projector_term(beta_s, d, pr.positions, gk)
+    phases = np.exp(-1j * (gk @ pr.positions.T))
+    np.testing.assert_allclose(term_d.phases.cpu().numpy(), phases, rtol=0, atol=1e-14)
+    term_h = FactorizedProjectorTerm(np.asfortranarray(pr.beta_species), np.asfortranarray(phases),
+                                     d)
+    term_p = packed_projector_term(beta_s, [d] * pr.natom, pr.positions, gk)
+    kw = dict(real_potential=s.real_potential)
+    ham_d = PlaneWaveHamiltonian(s.basis, s.potential_g, (term_d,), **kw)
+    ham_h = PlaneWaveHamiltonian(s.basis, s.potential_g, (term_h,), **kw)
+    ham_p = PlaneWaveHamiltonian(s.basis, s.potential_g, (term_p,), **kw)
+    x = s.trial[:, :12]
+    ref = ham_h.apply(x)
+    got = ham_d.apply(x)
+    assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+    # the device factorized and device packed forms build the same blocks
+    np.testing.assert_array_equal(got, ham_p.apply(x))
+    np.testing.assert_allclose(ham_d.diagonal, ham_h.diagonal, rtol=0, atol=1e-12)
+    with pytest.raises(ValueError, match="coupling"):
+        factorized_projector_term(beta_s, np.eye(3), pr.positions, gk)
