@@ -137,7 +137,8 @@ int qepw_column_squared_norms(int precision, const void* v, int64_t ldv, int64_t
                               int nb, double* out, void* stream);
 
 /*
- * out[b] = sum_g conj(l[g, b]) r[g, b]  (device, nb (re, im) double pairs).
+ * out[b] = sum_g conj(l[g, b]) r[g, b]  (device, nb (re, im) double pairs);
+ * ldr = 0 pairs every column of l with the single column r (l^H r).
  * Replaces: column_inner_products (qepy_pw/_native_fft.pyx:1241-1291).
  */
 int qepw_column_inner_products(int precision, const void* l, int64_t ldl, const void* r,

@@ -375,7 +375,7 @@ int qepw_column_squared_norms(int precision, const void* v, int64_t ldv, int64_t
 int qepw_column_inner_products(int precision, const void* l, int64_t ldl, const void* r,
                                int64_t ldr, int64_t nrows, int nb, double* out,
                                void* stream) {
-  if (nb > 0 && (!l || !r || !out || ldl < nrows || ldr < nrows))
+  if (nb > 0 && (!l || !r || !out || ldl < nrows || (ldr < nrows && ldr != 0)))
     return fail(QEPW_ERR_ARG, "qepw_column_inner_products: bad argument");
   return finish(qepw::column_inner_products(precision, l, ldl, r, ldr, nrows, nb, out,
                                             static_cast<hipStream_t>(stream)),

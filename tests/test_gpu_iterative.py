@@ -67,6 +67,14 @@ def test_per_column_kernels():
     ref = np.stack([coef[0, 0] * hist[0, 3] + coef[0, 1] * hist[1, 3],
                     coef[1, 0] * hist[0, 1] + coef[1, 1] * hist[1, 1]])
     np.testing.assert_allclose(out.cpu().numpy(), ref, atol=1e-14)
+    # ldr = 0: every column of l against one vector (l^H r)
+    from qe_python_amd import _native as nat
+    ip = torch.empty(2 * nb, dtype=torch.float64).cuda()
+    nat.check(nat.load().qepw_column_inner_products(
+        nat.C128, xd.data_ptr(), npw, yd[1].data_ptr(), 0, npw, nb, ip.data_ptr(),
+        dv.stream_handle(xd.device)), "qepw_column_inner_products")
+    np.testing.assert_allclose(ip.cpu().numpy().view(np.complex128), x.conj() @ x[1],
+                               rtol=1e-13)
 
 
 def test_paro_reference_tests_through_a_host_operator():
