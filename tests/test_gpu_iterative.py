@@ -187,3 +187,22 @@ def test_argument_errors(small):
                          initial_vectors=np.repeat(s.trial[:, :1], 3, axis=1))
     with pytest.raises(ValueError, match="Ritz"):
         conjugate_gradient(ham.apply, ham.diagonal, 3, initial_vectors=x0, initial_is_ritz=True)
+
+
+def test_rmm_medium_matches_oracle():
+    """One RMM-DIIS call on the medium workload's 128 bands (the SCF's
+    per-iteration use): same H-application count and eigenvalues as the
+    oracle's call on the host."""
+    from qe_python_amd.diagonalization import PackedProjectorTerm, PlaneWaveHamiltonian, rmm_diis
+    s = make_system("medium")
+    beta, coup, diag = s.packed_term()
+    ham = PlaneWaveHamiltonian(s.basis, s.potential_g, (PackedProjectorTerm(beta, coup, diag),),
+                               real_potential=s.real_potential)
+    term = SimpleNamespace(beta=beta, coupling=coup, diagonal=diag)
+    ref_ham = OracleHamiltonian(s.basis.indices, s.basis.kinetic, s.shape, s.real_potential,
+                                (term,), potential_average=float(np.real(s.potential_g.flat[0])))
+    x0 = s.trial[:, :128]
+    got = rmm_diis(ham.apply, ham.diagonal, 128, initial_vectors=x0, operator_into=ham.apply_into,
+                   tolerance=1e-8)
+    ref = oit.rmm_diis(ref_ham.apply, ref_ham.diagonal, 128, initial_vectors=x0, tolerance=1e-8)
+    check_against_oracle(got, ref, atol=1e-10)
