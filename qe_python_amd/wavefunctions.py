@@ -150,8 +150,9 @@ class WavefunctionBuffer:
         if not isinstance(coefficients, torch.Tensor) or coefficients.device.type == "cpu":
             if perm is not None:
                 raise ValueError("perm applies to device-order device blocks only")
-            matrix = np.asarray(coefficients.numpy() if isinstance(coefficients, torch.Tensor)
-                                else coefficients, dtype=np.complex128)
+            if isinstance(coefficients, torch.Tensor):
+                coefficients = coefficients.detach().resolve_conj().resolve_neg().numpy()
+            matrix = np.asarray(coefficients, dtype=np.complex128)
             if matrix.shape != (rows, self.number_of_bands):
                 raise QEInputError(f"wavefunction record has shape {matrix.shape}, expected "
                                    f"{(rows, self.number_of_bands)}", routine="save_buffer")
@@ -171,9 +172,8 @@ class WavefunctionBuffer:
                 raise QEInputError(f"device block has shape {tuple(t.shape)}, expected "
                                    f"{(self.number_of_bands, rows)}", routine="save_buffer")
             block, direction, pdev = t, 1, perm
-        block = block.resolve_conj().resolve_neg()
-        if block.dtype != torch.complex128:
-            raise ValueError("wavefunction records are complex128")
+        # records are complex128, as the reference's np.asarray(..., complex128)
+        block = block.detach().resolve_conj().resolve_neg().to(torch.complex128)
         if rows > 1 and block.stride(1) != 1:
             block = block.contiguous()
         dev, host = self._stage(t.device)
