@@ -228,6 +228,85 @@ __global__ __launch_bounds__(256) void k_reduce_splits(
   C[row + col * ldc] = out;
 }
 
+// One-column products (n = 1: a single band's projections and projector
+// sums) as matrix-vector passes instead of MFMA tiles with one live column.
+// Both write split-K partials part[s * m + i] that k_reduce_splits sums in a
+// fixed order and scales by alpha, beta.
+//   k_gemv_n: sum_{j in split s} A[i + j lda] b_j, one thread per row i;
+//   k_gemv_c: sum_{g in split s} conj(A[g + i lda]) b_g, one workgroup per
+//             (i, s) with a fixed-order tree sum.
+// b_j = B[j bstride], conjugated when CB.
+template <typename T, bool CB>
+__global__ __launch_bounds__(256) void k_gemv_n(int64_t m, int64_t k, int64_t kchunk,
+                                                const cplx<T>* __restrict__ A, int64_t lda,
+                                                const cplx<T>* __restrict__ B, int64_t bstride,
+                                                cplx<T>* __restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  const int64_t j0 = (int64_t)blockIdx.y * kchunk;
+  const int64_t j1 = j0 + kchunk < k ? j0 + kchunk : k;
+  T sr = T(0), si = T(0);
+#pragma unroll 8
+  for (int64_t j = j0; j < j1; ++j) {
+    const cplx<T> a = A[i + j * lda];
+    const cplx<T> b = B[j * bstride];
+    const T bi = CB ? -b.y : b.y;
+    sr += a.x * b.x - a.y * bi;
+    si += a.x * bi + a.y * b.x;
+  }
+  part[(int64_t)blockIdx.y * m + i] = mk<T>(sr, si);
+}
+
+template <typename T, bool CB>
+__global__ __launch_bounds__(256) void k_gemv_c(int64_t m, int64_t k, int64_t kchunk,
+                                                const cplx<T>* __restrict__ A, int64_t lda,
+                                                const cplx<T>* __restrict__ B, int64_t bstride,
+                                                cplx<T>* __restrict__ part) {
+  __shared__ T shr[256], shi[256];
+  const int64_t i = blockIdx.x;
+  const int64_t g0 = (int64_t)blockIdx.y * kchunk;
+  const int64_t g1 = g0 + kchunk < k ? g0 + kchunk : k;
+  const cplx<T>* col = A + i * lda;
+  T sr = T(0), si = T(0);
+  for (int64_t g = g0 + threadIdx.x; g < g1; g += 256) {
+    const cplx<T> a = col[g];
+    const cplx<T> b = B[g * bstride];
+    const T bi = CB ? -b.y : b.y;
+    sr += a.x * b.x + a.y * bi;
+    si += a.x * bi - a.y * b.x;
+  }
+  const int tid = threadIdx.x;
+  shr[tid] = sr;
+  shi[tid] = si;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+      shr[tid] += shr[tid + w];
+      shi[tid] += shi[tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) part[(int64_t)blockIdx.y * m + i] = mk<T>(shr[0], shi[0]);
+}
+
+// Splits of a one-column product: about 1024 workgroups in all, each split of
+// K at least 8 columns of A (k_gemv_n) or 1024 rows (k_gemv_c).
+struct GemvPlan {
+  int64_t splits;
+  int64_t kchunk;
+};
+
+static GemvPlan gemv_plan(bool ca, int64_t m, int64_t k) {
+  const int64_t blocks = ca ? m : (m + 255) / 256;
+  const int64_t most = ca ? (k + 1023) / 1024 : (k + 7) / 8;
+  int64_t splits = (1024 + blocks - 1) / blocks;
+  if (splits > most) splits = most;
+  if (splits > 65535) splits = 65535;
+  if (splits < 1) splits = 1;
+  const int64_t kchunk = (k + splits - 1) / splits;
+  return {(k + kchunk - 1) / kchunk, kchunk};
+}
+
 struct SplitPlan {
   int64_t splits;
   int64_t kchunk;
@@ -385,6 +464,9 @@ int herm_product(int n, const void* x, const void* y, const void* d, void* c, in
 }
 
 size_t gemm_workspace_bytes(int precision, int op, int64_t m, int64_t n, int64_t k) {
+  const size_t elem = precision == QEPW_C128 ? sizeof(double2) : sizeof(float2);
+  if (n == 1 && k > 0 && m > 0)
+    return (size_t)gemv_plan((op & QEPW_OP_C) != 0, m, k).splits * (size_t)m * elem;
   const SplitPlan sp = split_plan(op & QEPW_OP_C, m, n, k);
   if (sp.splits <= 1) return 0;
   const size_t c = precision == QEPW_C128 ? sizeof(double2) : sizeof(float2);
@@ -392,9 +474,46 @@ size_t gemm_workspace_bytes(int precision, int op, int64_t m, int64_t n, int64_t
 }
 
 template <typename T>
+static int gemv_t(int op, int64_t m, int64_t k, const double* alpha, const void* a, int64_t lda,
+                  const void* b, int64_t ldb, const double* beta, void* c, int64_t ldc,
+                  void* ws, size_t ws_bytes, hipStream_t stream) {
+  const bool ca = (op & QEPW_OP_C) != 0, cb = (op & QEPW_OP_BC) != 0;
+  const GemvPlan gp = gemv_plan(ca, m, k);
+  if (ws_bytes < (size_t)gp.splits * (size_t)m * sizeof(cplx<T>)) return QEPW_ERR_WORKSPACE;
+  // the single column of op(B): B[j] (k x 1), or conj(B[j ldb]) for its (1 x k) adjoint
+  const int64_t bstride = cb ? ldb : 1;
+  const cplx<T>* pa = static_cast<const cplx<T>*>(a);
+  const cplx<T>* pb = static_cast<const cplx<T>*>(b);
+  cplx<T>* part = static_cast<cplx<T>*>(ws);
+  if (ca) {
+    const dim3 grid((unsigned)m, (unsigned)gp.splits);
+    if (cb)
+      hipLaunchKernelGGL((k_gemv_c<T, true>), grid, dim3(256), 0, stream, m, k, gp.kchunk, pa,
+                         lda, pb, bstride, part);
+    else
+      hipLaunchKernelGGL((k_gemv_c<T, false>), grid, dim3(256), 0, stream, m, k, gp.kchunk, pa,
+                         lda, pb, bstride, part);
+  } else {
+    const dim3 grid((unsigned)((m + 255) / 256), (unsigned)gp.splits);
+    if (cb)
+      hipLaunchKernelGGL((k_gemv_n<T, true>), grid, dim3(256), 0, stream, m, k, gp.kchunk, pa,
+                         lda, pb, bstride, part);
+    else
+      hipLaunchKernelGGL((k_gemv_n<T, false>), grid, dim3(256), 0, stream, m, k, gp.kchunk, pa,
+                         lda, pb, bstride, part);
+  }
+  hipLaunchKernelGGL(k_reduce_splits<T>, dim3((unsigned)((m + 63) / 64)), dim3(256), 0, stream,
+                     m, (int64_t)1, (int)gp.splits, part, static_cast<cplx<T>*>(c), ldc,
+                     (T)alpha[0], (T)alpha[1], (T)beta[0], (T)beta[1]);
+  return hipGetLastError() == hipSuccess ? QEPW_OK : QEPW_ERR_HIP;
+}
+
+template <typename T>
 static int gemm_t(int op, int64_t m, int64_t n, int64_t k, const double* alpha,
                   const void* a, int64_t lda, const void* b, int64_t ldb, const double* beta,
                   void* c, int64_t ldc, void* ws, size_t ws_bytes, hipStream_t stream) {
+  if (n == 1 && k > 0)
+    return gemv_t<T>(op, m, k, alpha, a, lda, b, ldb, beta, c, ldc, ws, ws_bytes, stream);
   const SplitPlan sp = split_plan(op & QEPW_OP_C, m, n, k);
   if (sp.splits > 65535) return QEPW_ERR_UNSUPPORTED;
   const size_t need = sp.splits > 1 ? (size_t)sp.splits * m * n * sizeof(cplx<T>) : 0;
