@@ -35,7 +35,10 @@ def rand(rng, *shape):
 @pytest.mark.parametrize("op", ["N", "C", "NB", "CB"])
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (5, 7, 3), (64, 64, 16), (65, 130, 37),
                                    (256, 128, 23847), (17, 300, 4000), (2945, 8, 32),
-                                   (384, 384, 384), (500, 130, 257), (3, 2, 0)])
+                                   (384, 384, 384), (500, 130, 257), (3, 2, 0),
+                                   # one output column: the matrix-vector kernels
+                                   (23847, 1, 256), (256, 1, 23847), (300, 1, 5000),
+                                   (1, 1, 3000), (70000, 1, 3), (7, 1, 0)])
 def test_gemm_matches_numpy(prec, op, m, n, k):
     from qe_python_amd import _native as nat
     from qe_python_amd import device as dv
@@ -76,6 +79,21 @@ def test_gemm_split_k_is_deterministic():
         tc = torch.empty((160, 128), dtype=torch.complex128, device=DEV)
         dv.gemm(nat.OP_C, 128, 160, 23847, 1.0, ta, tb, 0.0, tc, ws)
         outs.append(tc.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_gemv_beta_zero_ignores_nan_output_and_is_deterministic():
+    from qe_python_amd import _native as nat
+    from qe_python_amd import device as dv
+    rng = np.random.default_rng(10)
+    a, b = rand(rng, 23847, 256), rand(rng, 23847, 1)
+    ws = dv.Workspace(torch.device(DEV))
+    outs = []
+    for _ in range(2):
+        tc = torch.full((1, 256), float("nan"), dtype=torch.complex128, device=DEV)
+        dv.gemm(nat.OP_C, 256, 1, 23847, 1.0, dev_block(a), dev_block(b), 0.0, tc, ws)
+        outs.append(host_block(tc))
+    np.testing.assert_allclose(outs[0], a.conj().T @ b, rtol=1e-12, atol=1e-12)
     assert np.array_equal(outs[0], outs[1])
 
 
