@@ -502,7 +502,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
-              "(the path would shard by k-points / bands; see DESIGN.md)", file=sys.stderr)
+              "(several GPUs would take whole k-points each, then bands)", file=sys.stderr)
         return 2
     if args.fft_probe:
         return fft_probe(args.fft_probe)
