@@ -119,30 +119,12 @@ class _Blocks:
         """x -= ref (ref^H x); each (r, y) in ``others`` gets y -= r (ref^H x)."""
         if not ref.shape[0] or not x.shape[0]:
             return
-        if x.shape[0] == 1:
-            self._project_out_vector(ref, x, *others)
-            return
+        # one vector (CG's projections): qepw_gemm runs these one-column
+        # products as matrix-vector passes
         p = self.gram_dev(ref, x)
         dv.gemm(nat.OP_N, self.npw, x.shape[0], ref.shape[0], -1.0, ref, p, 1.0, x, self.ws)
         for r, y in others:
             dv.gemm(nat.OP_N, self.npw, x.shape[0], ref.shape[0], -1.0, r, p, 1.0, y, self.ws)
-
-    def _project_out_vector(self, ref, x, *others) -> None:
-        """project_out for one vector (CG's band-by-band projections): the
-        coefficients ref^H x as broadcast column inner products, the
-        updates as one-column history combinations -- matrix-vector passes
-        instead of 64 x 64 MFMA tiles with one live column."""
-        n = int(ref.shape[0])
-        p = torch.empty(2 * n, dtype=torch.float64, device=self.device)
-        rc = nat.load().qepw_column_inner_products(
-            nat.C128, ref.data_ptr(), dv.ld_of(ref), x.data_ptr(), 0, self.npw, n, p.data_ptr(),
-            dv.stream_handle(self.device))
-        nat.check(rc, "qepw_column_inner_products")
-        coef = p.view(torch.complex128).view(1, n)
-        tmp = torch.empty_like(x)
-        for r, y in ((ref, x),) + tuple(others):
-            dv.history_combine(r.unsqueeze(1), n, None, coef, tmp, self.npw)
-            self.axpby(tmp, -1.0, y, None)
 
     def dots(self, *pairs) -> list:
         """Per-column <l|r> of each (l, r) pair (complex host arrays), with one
