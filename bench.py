@@ -23,6 +23,11 @@ The JSON line carries:
   reference) on the same inputs and kwargs, on <= 16 host threads, for a
   bounded time budget; if it does not finish, its time is extrapolated by
   the H-application count of the device solve.
+* ``density``, ``projectors``, ``iterative`` and ``wavefunction_io`` (unless
+  ``--no-legs``): the SCF's density accumulation, the per-k projector
+  assembly, the CG / ParO / RMM-DIIS solvers and the wavefunction working
+  file on the same workload, each with its oracle timed on the host beside
+  it (unless ``--no-cpu``).
 """
 
 from __future__ import annotations
@@ -60,6 +65,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 24-problem uneven batch leg")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the density, projector, iterative-solver and "
+                         "wavefunction-file legs (the Davidson solves only)")
     ap.add_argument("--batch-workers", type=int, default=2)
     ap.add_argument("--breakdown", action="store_true",
                     help="print a per-label timer breakdown of one extra solve to stderr")
@@ -625,22 +633,25 @@ def main():
             ham.timers = None
     if not args.no_batch:
         line["batch"] = batch_leg(args, device)
-    density, d_coeff, d_weights, d_grid = density_leg(system, device)
-    line["density"] = density
-    projectors, p_inputs, p_packed = projector_leg(system, device)
-    line["projectors"] = projectors
-    iterative, rmm_result = iterative_leg(system, ham, trial)
-    line["iterative"] = iterative
-    line["wavefunction_io"] = wavefunction_io_leg(system, device, not args.no_cpu)
+    legs = not args.no_legs
+    if legs:
+        density, d_coeff, d_weights, d_grid = density_leg(system, device)
+        line["density"] = density
+        projectors, p_inputs, p_packed = projector_leg(system, device)
+        line["projectors"] = projectors
+        iterative, rmm_result = iterative_leg(system, ham, trial)
+        line["iterative"] = iterative
+        line["wavefunction_io"] = wavefunction_io_leg(system, device, not args.no_cpu)
     if not args.no_cpu:
         happ = head["hamiltonian_applications"][-1]
         line["cpu_baseline"] = cpu_baseline(system, kwargs, happ, args.cpu_budget)
-        line["density"].update(density_cpu_baseline(system, d_coeff, d_weights, d_grid,
-                                                    density["ms_per_call"]))
-        line["projectors"].update(projector_cpu_baseline(system, p_inputs, p_packed,
-                                                         projectors["ms_per_call"]))
-        line["iterative"]["rmm_diis"].update(iterative_cpu_baseline(
-            system, system.trial, rmm_result, iterative["rmm_diis"]["ms_per_call"]))
+        if legs:
+            line["density"].update(density_cpu_baseline(system, d_coeff, d_weights, d_grid,
+                                                        density["ms_per_call"]))
+            line["projectors"].update(projector_cpu_baseline(system, p_inputs, p_packed,
+                                                             projectors["ms_per_call"]))
+            line["iterative"]["rmm_diis"].update(iterative_cpu_baseline(
+                system, system.trial, rmm_result, iterative["rmm_diis"]["ms_per_call"]))
         if head["value"] > 0:
             line["config"]["gpu_over_cpu"] = head["value"] / line["cpu_baseline"]["value"]
     print(json.dumps(line))
