@@ -410,7 +410,11 @@ def parallel_orbital(operator, diagonal, number_of_roots, *, initial_vectors,
     [basis, corrections]; converged on Ritz-value changes below
     ``tolerance``.  Like the reference it recomputes H psi and ignores
     ``initial_applied`` / ``initial_eigenvalues`` / ``initial_is_ritz``,
-    ``occupied_roots`` and ``full_accuracy``."""
+    ``occupied_roots`` and ``full_accuracy``.  For one or two roots the
+    reference's first candidate list takes basis columns it has not filled
+    (``np.empty`` storage); here they are zero, so such a call ends in a
+    ``LinAlgError`` from the singular overlap instead of reading whatever
+    memory held."""
     if tolerance <= 0.0 or max_iterations < 1:
         raise ValueError("invalid ParO diagonalization controls")
     del initial_applied, initial_eigenvalues, initial_is_ritz, occupied_roots, full_accuracy
