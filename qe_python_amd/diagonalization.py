@@ -56,6 +56,9 @@ REDUCED_EIGENSOLVER = "device"
 # residual norms below 1e-5, about ten times the FP32 noise floor of both
 # (unit roundoff 6e-8 times |H| ~ 15 Ha for a 30 Ry cutoff)
 MIXED_SWITCH = (1.0e-5, 1.0e-5)
+# host Hermitian eigensolves of the complex64 phase in single precision
+# (LAPACK cheevd instead of zheevd)
+MIXED_SINGLE_EIGH = True
 # a complex64 phase whose largest unconverged residual is within 100x of the
 # switch residual and has not halved over this many iterations sits at the
 # FP32 floor and hands over (far above the floor, slow progress is the
@@ -861,8 +864,9 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
         HV[:active].copy_(_upload_block(given, perm_dev, npw, device, dtype))
         happ = 0
     # the reduced pencil Hs = V^H HV, Ss = V^H V lives on the device
+    single = MIXED_SINGLE_EIGH and dtype == torch.complex64
     rr = DeviceRayleighRitz(maximum_subspace, number_of_roots, device, ws,
-                            spectral=ctx.spectral)
+                            spectral=ctx.spectral, host_single=single)
     diag_dev = ctx.diag_dev
 
     def update_projected(first):
@@ -1093,7 +1097,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
                     # root count restarts too often for the slow top roots
                     old_theta = rr.theta
                     rr = DeviceRayleighRitz(maximum_subspace, number_of_roots, device, ws,
-                                            spectral=ctx.spectral)
+                                            spectral=ctx.spectral, host_single=single)
                     if ITERATION_TRACE is not None:
                         ITERATION_TRACE.append(("lock", iteration, int(locked.shape[0]),
                                                 number_of_roots, maximum_subspace))

@@ -88,7 +88,9 @@ class DeviceRayleighRitz:
     """The device-resident reduced problem of one Davidson solve."""
 
     def __init__(self, maxsub: int, nroots: int, device, ws: dv.Workspace, *,
-                 spectral: bool = True, extra: int = 16, check: bool = False):
+                 spectral: bool = True, extra: int = 16, check: bool = False,
+                 host_single: bool = False):
+        self.host_single = host_single   # host Hermitian eigensolves in complex64
         self.maxsub = int(maxsub)
         self.nroots = int(nroots)
         self.device = device
@@ -274,7 +276,7 @@ class DeviceRayleighRitz:
         k = self.nroots
         if n <= HOST_DENSE_MAX:
             a = self.host_block(self.A, n)
-            w, v = np.linalg.eigh(0.5 * (a + a.conj().T))
+            w, v = self._host_eigh(0.5 * (a + a.conj().T))
             U = dv.upload(v[:, :k].T, self.device)
             w = torch.from_numpy(w)
         else:
@@ -292,6 +294,15 @@ class DeviceRayleighRitz:
         self.theta = values
         self.sigma = None
         return values[:k].copy(), coef
+
+    def _host_eigh(self, a: np.ndarray):
+        """All eigenpairs of a Hermitian host matrix (LAPACK heevd), in
+        complex64 when ``host_single`` (the complex64 phase of the mixed form,
+        whose vectors are complex64 anyway), returned in FP64."""
+        if not self.host_single:
+            return np.linalg.eigh(a)
+        w, v = sla.eigh(a.astype(np.complex64), driver="evd", check_finite=False)
+        return w.astype(np.float64), v.astype(np.complex128)
 
     def _can_keep_shift(self) -> bool:
         theta, k = self.theta, self.nroots
@@ -476,7 +487,7 @@ class DeviceRayleighRitz:
         self._gemm(nat.OP_C, mp, mp, n, 1.0, Q, AQ, 0.0, B)
         bt = B.cpu().numpy().T
         bt = 0.5 * (bt + bt.conj().T)
-        w, y = np.linalg.eigh(bt)
+        w, y = self._host_eigh(bt)
         if len(w) < k:
             return None
         y_dev = dv.upload(y[:, :k].T, self.device)
