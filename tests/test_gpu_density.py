@@ -63,3 +63,25 @@ def test_density_device_tensors_and_errors():
                            potential_average=float(np.real(s.potential_g.flat[0])))
     got, want = ham.apply(coeff), oh.apply(coeff)
     assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+
+
+def test_density_at_a_general_kpoint():
+    """A k-point basis shifts the sphere (other sticks and active columns): the
+    density of its bands against the oracle, and the SCF identity
+    sum_r V(r) rho(r) / N = sum_b w_b <psi_b|V_loc psi_b> with the device
+    Hamiltonian's local part."""
+    from qe_python_amd.basis import LocalPotentialWorkspace
+    s = make_system("medium", kpoint=np.array([0.21, 0.07, 0.33]))
+    nb = 16
+    rng = np.random.default_rng(43)
+    coeff = s.trial[:, :nb]
+    w = rng.uniform(0.1, 2.0, nb)
+    ref = accumulate_density(coeff, w, linear_slots(s.basis.indices, s.shape), s.shape)
+    ws = LocalPotentialWorkspace(s.basis.indices, s.shape)
+    rho = np.zeros(s.shape)
+    ws.accumulate_density(rho, coeff, w)
+    np.testing.assert_allclose(rho, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    local = OracleHamiltonian(s.basis.indices, np.zeros(s.npw), s.shape, s.real_potential)
+    expect = np.sum(w * np.real(np.einsum("gb,gb->b", coeff.conj(), local.apply(coeff))))
+    got = np.sum(s.real_potential * rho) / rho.size
+    assert abs(got - expect) <= 1e-11 * abs(expect)
