@@ -150,3 +150,44 @@ def test_factorized_term_on_the_device():
     np.testing.assert_allclose(ham_d.diagonal, ham_h.diagonal, rtol=0, atol=1e-12)
     with pytest.raises(ValueError, match="coupling"):
         factorized_projector_term(beta_s, np.eye(3), pr.positions, gk)
+
+
+def test_medium_factorized_term_at_a_kpoint():
+    """The medium cell's 64 atoms x 4 channels (256 > 64 packed channels: the
+    reference SCF's factorized form) at a general k-point: projectors
+    assembled on the device from the Gaussian radial tables, factorized, in
+    a Hamiltonian applied to 8 bands, against the same Hamiltonian with the
+    host-packed analytic projectors (the interpolation's ~5e-9 relative
+    error bounds the difference)."""
+    import torch
+
+    from qe_python_amd.diagonalization import PackedProjectorTerm, PlaneWaveHamiltonian
+    from qe_python_amd.projectors import factorized_projector_term, projector_basis
+    s = make_system("medium", kpoint=np.array([0.21, 0.07, 0.33]))
+    pr = s.projectors
+    gk = s.basis.vectors
+    dq = 0.01
+    table = gaussian_table(s.volume, dq, int(np.floor(np.linalg.norm(gk, axis=1).max() / dq + 4.5)))
+    d = pr.couplings[0]
+    beta = projector_basis(gk, table, dq, [(0, 0, 0), (1, 1, 0), (1, 1, 1), (1, 1, 2)])
+    term_d = factorized_projector_term(beta, d, pr.positions, gk)
+    phases = np.exp(-1j * (gk @ pr.positions.T))
+    beta_h = np.concatenate([pr.beta_species * phases[:, a, None] for a in range(pr.natom)],
+                            axis=1)
+    coup_h = np.kron(np.eye(pr.natom), d)
+    diag_h = pr.natom * np.real(np.einsum("gi,ij,gj->g", pr.beta_species, d,
+                                          pr.beta_species.conj()))
+    term_h = PackedProjectorTerm(np.asfortranarray(beta_h), coup_h, diag_h)
+    kw = dict(real_potential=s.real_potential)
+    ham_d = PlaneWaveHamiltonian(s.basis, s.potential_g, (term_d,), **kw)
+    ham_h = PlaneWaveHamiltonian(s.basis, s.potential_g, (term_h,), **kw)
+    x = s.trial[:, :8]
+    ref = ham_h.apply(x)
+    got = ham_d.apply(x)
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"medium factorized at k: relative difference {rel:.2e}", flush=True)
+    assert rel <= 1e-8
+    # the diagonal is quadratic in beta: twice the interpolation's error
+    np.testing.assert_allclose(ham_d.diagonal, ham_h.diagonal, rtol=0,
+                               atol=2e-8 * np.abs(ham_h.diagonal).max())
+    del torch
