@@ -85,19 +85,29 @@ def require_device(device=None) -> torch.device:
     return dev
 
 
-def ld_of(t: torch.Tensor) -> int:
-    if t.dim() != 2:
-        raise ValueError("device matrix must be a (ncols, ld) tensor with unit inner stride")
-    s0, s1 = t.shape
-    st0, st1 = t.stride()
-    if s0 > 1 and s1 > 1 and st1 != 1:
-        raise ValueError("device matrix must be a (ncols, ld) tensor with unit inner stride")
+_NOT_MATRIX = "device matrix must be a (ncols, ld) tensor with unit inner stride"
+
+
+def _matrix(t: torch.Tensor):
+    """(ld, ncols, nrows) of a (ncols, ld) device matrix view, one pass over
+    its metadata (the GEMM wrapper runs ~1000 times per Davidson solve)."""
+    shape, stride = t.shape, t.stride()
+    if len(shape) != 2:
+        raise ValueError(_NOT_MATRIX)
+    s0, s1 = shape
+    st0, st1 = stride
+    if st1 != 1 and s0 > 1 and s1 > 1:
+        raise ValueError(_NOT_MATRIX)
     if t.is_conj() or t.is_neg():
         # a lazy conjugate / negative view: its memory holds the plain values
         raise ValueError("device matrix is a lazy conj/neg view; resolve it first")
     if s0 > 1:
-        return max(1, st0)
-    return max(1, s1, st0)
+        return (st0 if st0 > 1 else 1), s0, s1
+    return max(1, s1, st0), s0, s1
+
+
+def ld_of(t: torch.Tensor) -> int:
+    return _matrix(t)[0]
 
 
 class Workspace:
@@ -133,14 +143,17 @@ def gemm(op_a: int, m: int, n: int, k: int, alpha, a: torch.Tensor, b: torch.Ten
     if a.dtype != dt or b.dtype != dt:
         raise ValueError("gemm operands must share one complex dtype")
     prec = precision_code(dt)
-    lda, ldb, ldc = ld_of(a), ld_of(b), ld_of(c)
+    lda, a0, a1 = _matrix(a)
+    ldb, b0, b1 = _matrix(b)
+    ldc, c0, c1 = _matrix(c)
     if m > 0 and n > 0 and k > 0:
-        a_shape = (m, k) if op_a & nat.OP_C else (k, m)
-        b_shape = (k, n) if op_a & nat.OP_BC else (n, k)
-        for name, t, (cols, rows) in (("a", a, a_shape), ("b", b, b_shape), ("c", c, (n, m))):
-            if t.shape[0] < cols or t.shape[1] < rows:
-                raise ValueError(f"gemm operand {name} of shape {tuple(t.shape)} is smaller "
-                                 f"than the {rows} x {cols} matrix the sizes name")
+        ac, ar = (m, k) if op_a & nat.OP_C else (k, m)
+        bc, br = (k, n) if op_a & nat.OP_BC else (n, k)
+        if a0 < ac or a1 < ar or b0 < bc or b1 < br or c0 < n or c1 < m:
+            for name, t, cols, rows in (("a", a, ac, ar), ("b", b, bc, br), ("c", c, n, m)):
+                if t.shape[0] < cols or t.shape[1] < rows:
+                    raise ValueError(f"gemm operand {name} of shape {tuple(t.shape)} is "
+                                     f"smaller than the {rows} x {cols} matrix the sizes name")
     key = (prec, op_a, m, n, k)
     need = _GEMM_WORKSPACE.get(key)
     if need is None:
@@ -152,7 +165,8 @@ def gemm(op_a: int, m: int, n: int, k: int, alpha, a: torch.Tensor, b: torch.Ten
     rc = lib.qepw_gemm(prec, op_a, m, n, k, nat.complex_pair(alpha), a.data_ptr(), lda,
                        b.data_ptr(), ldb, nat.complex_pair(beta), c.data_ptr(), ldc,
                        buf.data_ptr(), buf.numel(), stream_handle(c.device))
-    nat.check(rc, "qepw_gemm")
+    if rc:
+        nat.check(rc, "qepw_gemm")
 
 
 def column_squared_norms(v: torch.Tensor, nrows: int, out: torch.Tensor | None = None) -> torch.Tensor:

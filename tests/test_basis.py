@@ -65,3 +65,27 @@ def test_potential_matrix_is_convolution():
     idx = np.array([[0, 0, 0], [1, 0, 0], [0, -1, 2]])
     mat = potential_matrix(vg, idx)
     assert mat[1, 2] == vg[(1 - 0) % 6, (0 + 1) % 6, (0 - 2) % 6]
+
+
+def test_gemm_wrapper_guards_raise_before_the_library():
+    """The GEMM wrapper's operand checks (undersized views, 3-D tensors,
+    non-unit inner strides, lazy conj views, mixed dtypes) raise ValueError
+    on the host before any device call."""
+    import torch
+
+    from qe_python_amd import _native as nat
+    from qe_python_amd import device as dv
+    z = lambda *s, dt=torch.complex128: torch.zeros(s, dtype=dt)  # noqa: E731
+    with pytest.raises(ValueError, match="smaller"):
+        dv.gemm(nat.OP_N, 10, 4, 3, 1.0, z(3, 9), z(4, 3), 0.0, z(4, 10), None)
+    with pytest.raises(ValueError, match="smaller"):
+        dv.gemm(nat.OP_C, 10, 4, 30, 1.0, z(10, 30), z(4, 30), 0.0, z(3, 10), None)
+    with pytest.raises(ValueError, match="unit inner stride"):
+        dv.gemm(nat.OP_N, 4, 4, 4, 1.0, z(2, 4, 4), z(4, 4), 0.0, z(4, 4), None)
+    with pytest.raises(ValueError, match="unit inner stride"):
+        dv.gemm(nat.OP_N, 4, 4, 4, 1.0, z(4, 8)[:, ::2], z(4, 4), 0.0, z(4, 4), None)
+    with pytest.raises(ValueError, match="lazy conj"):
+        dv.gemm(nat.OP_N, 4, 4, 4, 1.0, z(4, 4).conj(), z(4, 4), 0.0, z(4, 4), None)
+    with pytest.raises(ValueError, match="one complex dtype"):
+        dv.gemm(nat.OP_N, 4, 4, 4, 1.0, z(4, 4, dt=torch.complex64), z(4, 4), 0.0, z(4, 4), None)
+    assert dv.ld_of(z(5, 7)) == 7 and dv.ld_of(z(1, 7)) == 7 and dv.ld_of(z(5, 7)[:, :3]) == 7
