@@ -53,15 +53,18 @@ def fft_phase_bytes(nb, npw, nst, n3, ngrid, complex_bytes):
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--tol-ry", type=float, default=1e-8)
     ap.add_argument("--tol-factor", type=float, default=0.01,
                     help="Davidson tolerance as a fraction of the target (both forms)")
     ap.add_argument("--form", choices=("both", "mixed", "fp64"), default="both",
                     help="forms to time; the first (mixed for 'both') is the headline value")
-    ap.add_argument("--cpu-budget", type=float, default=20.0,
-                    help="seconds of oracle Davidson work for the CPU baseline")
+    ap.add_argument("--cpu-budget", type=float, default=60.0,
+                    help="seconds after which one oracle Davidson solve of the CPU "
+                         "baseline is stopped and scaled")
+    ap.add_argument("--cpu-repeats", type=int, default=3,
+                    help="complete oracle solves timed for the CPU baseline (median)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 24-problem uneven batch leg")
@@ -122,11 +125,57 @@ class _Budget(list):
             raise _Budget.Spent()
 
 
-def cpu_baseline(system, kwargs, device_happ, budget):
+def _sha256(array):
+    import hashlib
+
+    import numpy as np
+    return hashlib.sha256(np.ascontiguousarray(array).tobytes()).hexdigest()
+
+
+def host_description():
+    """lscpu model, sockets and NUMA nodes, the BLAS and FFT providers."""
+    import subprocess
+
+    import scipy
+    from threadpoolctl import threadpool_info
+
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            key, _, value = line.partition(":")
+            if key.strip() in ("Model name", "Socket(s)", "NUMA node(s)", "CPU(s)",
+                               "Thread(s) per core"):
+                info[key.strip()] = value.strip()
+    except (OSError, subprocess.SubprocessError):
+        info["lscpu"] = "unavailable"
+    info["blas"] = [f"{p.get('internal_api')} {p.get('version')} ({p.get('num_threads')} threads)"
+                    for p in threadpool_info() if p.get("user_api") == "blas"]
+    info["fft"] = f"scipy.fft (pocketfft), scipy {scipy.__version__}"
+    return info
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may run on,
+    within OMP_NUM_THREADS when that is set (the GPU box's CPU share)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        n = min(n, int(omp))
+    return max(1, int(n or 1))
+
+
+def cpu_baseline(system, kwargs, device_happ, budget, repeats):
+    """The oracle Davidson (the NumPy/SciPy restatement of the reference
+    solver) on the same inputs and kwargs: one short warm-up solve (two
+    iterations: FFT plans, BLAS threads), then ``repeats`` complete solves
+    timed one by one; the value is 1 / their median.  A solve that runs
+    past ``budget`` seconds is stopped and its time scaled by the device
+    solve's H-application count."""
     import numpy as np
     from types import SimpleNamespace
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = cpu_threads()
     os.environ["ORACLE_FFT_WORKERS"] = str(threads)
     from threadpoolctl import threadpool_limits
 
@@ -138,62 +187,94 @@ def cpu_baseline(system, kwargs, device_happ, budget):
                                 system.real_potential,
                                 (SimpleNamespace(beta=beta, coupling=coup, diagonal=diag),),
                                 potential_average=float(np.real(system.potential_g.flat[0])))
-    trace = _Budget(budget)
-    t0 = time.perf_counter()
-    finished = True
+    times, samples = [], []
     with threadpool_limits(limits=threads):
-        try:
-            odav.davidson(ham.apply, ham.diagonal, system.nroots,
-                          initial_vectors=system.trial, trace=trace, **kwargs)
-        except _Budget.Spent:
-            finished = False
-    elapsed = time.perf_counter() - t0
-    happ = ham.applications
-    if finished:
-        solve_s = elapsed
-        sample = (f"full oracle Davidson solve of the same workload ({len(trace)} iterations, "
-                  f"{happ} H applications) in {elapsed:.1f} s")
-    else:
-        solve_s = elapsed * device_happ / max(happ, 1)
-        sample = (f"first {len(trace)} iterations of the oracle Davidson on the same workload "
-                  f"({happ} H applications, {elapsed:.1f} s), scaled to the device solve's "
-                  f"{device_happ} H applications")
-    return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": sample}
+        odav.davidson(ham.apply, ham.diagonal, system.nroots, initial_vectors=system.trial,
+                      **{**kwargs, "max_iterations": 2})
+        host = host_description()
+        for _ in range(repeats):
+            ham.applications = 0
+            trace = _Budget(budget)
+            t0 = time.perf_counter()
+            finished = True
+            try:
+                odav.davidson(ham.apply, ham.diagonal, system.nroots,
+                              initial_vectors=system.trial, trace=trace, **kwargs)
+            except _Budget.Spent:
+                finished = False
+            elapsed = time.perf_counter() - t0
+            happ = ham.applications
+            if finished:
+                times.append(elapsed)
+                samples.append(f"{len(trace)} iterations / {happ} H applications in "
+                               f"{elapsed:.1f} s")
+            else:
+                times.append(elapsed * device_happ / max(happ, 1))
+                samples.append(f"first {len(trace)} iterations ({happ} H applications) in "
+                               f"{elapsed:.1f} s, scaled to {device_happ}")
+    median = float(np.median(times))
+    return {"value": 1.0 / median, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle Davidson of the same workload and kwargs, warm-up of 2 "
+                       f"iterations then {repeats} complete solves, median: "
+                       + "; ".join(samples)),
+            "seconds": times, "host": host,
+            "inputs_sha256": {"trial": _sha256(system.trial),
+                              "real_potential": _sha256(system.real_potential),
+                              "indices": _sha256(system.basis.indices),
+                              "beta": _sha256(beta)}}
 
 
-def run_form(form, ham, system, trial, kwargs, oracle_values, tol_ha, steps, warmup):
-    """Warm up, then time ``steps`` complete solves of one form; returns the
-    per-form record and the FFT-phase HIP events of the timed solves."""
+def run_form(form, ham, system, kwargs, oracle_values, tol_ha, steps, warmup):
+    """Warm up, then time ``steps`` complete solves of one form one by one,
+    host in / host out: the trial block is the host array (copied to the
+    device inside the call) and the result's eigenvectors come back as a host
+    array.  Returns the per-form record (value = 1 / median solve time), the
+    FFT-phase HIP events of the timed solves, and the results."""
     import numpy as np
     import torch
 
     from qe_python_amd.diagonalization import davidson
 
-    def solve():
-        return davidson(ham.apply, ham.diagonal, system.nroots, initial_vectors=trial,
-                        operator_into=ham.apply_into, device_output=True, precision=form,
-                        **kwargs)
+    trial = system.trial
+
+    def solve(vectors=trial, device_output=False):
+        return davidson(ham.apply, ham.diagonal, system.nroots, initial_vectors=vectors,
+                        operator_into=ham.apply_into, device_output=device_output,
+                        precision=form, **kwargs)
 
     for _ in range(warmup):
         solve()
     torch.cuda.synchronize()
     ham.fft_events = []
-    results = []
-    t0 = time.perf_counter()
+    results, times = [], []
     for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         results.append(solve())
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
     events = ham.fft_events
     ham.fft_events = None
+    # secondary: the same solves with the trial block already on the device
+    # and the eigenvectors left there
+    dev_trial = torch.from_numpy(trial).to(ham.device)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solve(dev_trial, True)
+    torch.cuda.synchronize()
+    resident_ms = 1e3 * (time.perf_counter() - t0) / steps
     errors = [np.abs(r.eigenvalues - oracle_values) for r in results]
     valid = all(r.converged for r in results) and max(float(e.max()) for e in errors) <= tol_ha
+    median = float(np.median(times))
     record = {
         "form": form,
-        "value": steps / elapsed if valid else 0.0,
+        "value": 1.0 / median if valid else 0.0,
         "unit": "solves/s",
-        "ms_per_step": 1e3 * elapsed / steps,
+        "ms_per_step": 1e3 * median,
+        "ms_each": [1e3 * x for x in times],
+        "ms_mean": 1e3 * float(np.mean(times)),
+        "device_resident_ms_mean": resident_ms,
         "iterations": [r.iterations for r in results],
         "hamiltonian_applications": [r.hamiltonian_applications for r in results],
         "max_eigenvalue_error_ha": max(float(e.max()) for e in errors),
@@ -201,6 +282,93 @@ def run_form(form, ham, system, trial, kwargs, oracle_values, tol_ha, steps, war
         "valid": valid,
     }
     return record, events, results
+
+
+class _WallTimers:
+    """Wall time per reference timer label (the ``measure(label)`` interface
+    the solver calls; the SCF passes its own ``TimingRegistry``)."""
+
+    def __init__(self):
+        self.entries = {}
+
+    def measure(self, label):
+        timers = self
+
+        class _Scope:
+            def __enter__(self):
+                self.t0 = time.perf_counter()
+
+            def __exit__(self, *exc):
+                ms, calls = timers.entries.get(label, (0.0, 0))
+                timers.entries[label] = (ms + 1e3 * (time.perf_counter() - self.t0), calls + 1)
+                return False
+
+        return _Scope()
+
+
+# dense MFMA peaks of one MI355X (TFLOP/s)
+MFMA_PEAK = {"fp64": 78.6, "fp32": 157.3}
+GEMM_CLASSES = {"cegterg:over": "update_projected (H/S rows, V^H HV and V^H V)",
+                "cegterg:upda": "Ritz vectors and residuals (V C, HV C)",
+                "cegterg:orth": "complex64 corrections against the basis",
+                "cegterg:last": "restart block",
+                "h_psi:calbec": "calbec (beta^H psi)",
+                "add_vuspsi": "add_vuspsi (beta D becp)",
+                "cdiaghg": "reduced eigensolve (standardisation, sign iteration, range)"}
+
+
+def mfma_leg(ham, system, kwargs, forms):
+    """MFMA utilisation of the dense phases: one extra solve per form with a
+    HIP event pair around every complex GEMM (on its launch stream), grouped
+    by the reference timer label it runs under.  flops = 8 m n k per complex
+    product; the fraction is against the dense MFMA peak of the GEMM's
+    precision (78.6 TFLOP/s FP64, 157.3 TFLOP/s FP32).  ``largest`` lists the
+    three shapes with the most flops and their own rates."""
+    import torch
+
+    from qe_python_amd import device as dv
+    from qe_python_amd.diagonalization import davidson
+
+    out = {}
+    for form in forms:
+        dv.GEMM_TRACE = trace = []
+        try:
+            davidson(ham.apply, ham.diagonal, system.nroots, initial_vectors=system.trial,
+                     operator_into=ham.apply_into, precision=form, **kwargs)
+            torch.cuda.synchronize()
+        finally:
+            dv.GEMM_TRACE = None
+        groups, shapes = {}, {}
+        for label, dtype, m, n, k, a, b in trace:
+            prec = "fp64" if dtype == torch.complex128 else "fp32"
+            ms = a.elapsed_time(b)
+            flops = 8.0 * m * n * k
+            g = groups.setdefault((label, prec), [0, 0.0, 0.0])
+            g[0] += 1
+            g[1] += flops
+            g[2] += ms
+            s = shapes.setdefault((prec, m, n, k), [0, 0.0, 0.0])
+            s[0] += 1
+            s[1] += flops
+            s[2] += ms
+        classes = {}
+        for (label, prec), (calls, flops, ms) in sorted(groups.items(), key=lambda kv: -kv[1][1]):
+            rate = flops / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+            classes[f"{label} {prec}"] = {
+                "what": GEMM_CLASSES.get(label, label), "calls": calls, "gflop": flops / 1e9,
+                "ms": ms, "tflops": rate, "frac": rate / MFMA_PEAK[prec]}
+        largest = []
+        for (prec, m, n, k), (calls, flops, ms) in sorted(shapes.items(),
+                                                          key=lambda kv: -kv[1][1])[:3]:
+            rate = flops / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+            largest.append({"shape_mnk": [m, n, k], "precision": prec, "calls": calls,
+                            "ms_per_call": ms / calls, "tflops": rate,
+                            "frac": rate / MFMA_PEAK[prec]})
+        total_flops = sum(v[1] for v in groups.values())
+        total_ms = sum(v[2] for v in groups.values())
+        out[form] = {"classes": classes, "largest": largest, "gemm_ms": total_ms,
+                     "gemm_gflop": total_flops / 1e9}
+    return out
 
 
 def batch_leg(args, device):
@@ -519,7 +687,6 @@ def main():
 
     from qe_python_amd.diagonalization import PackedProjectorTerm, PlaneWaveHamiltonian, davidson
     from qe_python_amd.synthetic import make_system
-    from qe_python_amd.timing import TimingRegistry
 
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
@@ -531,7 +698,6 @@ def main():
     ham = PlaneWaveHamiltonian(system.basis, system.potential_g,
                                (PackedProjectorTerm(beta, coup, diag),),
                                real_potential=system.real_potential, device=device)
-    trial = torch.from_numpy(system.trial).to(device)
     tol_ha = 0.5 * args.tol_ry
     # Davidson stops on |delta e| < tolerance, which does not bound the error
     # itself: both forms run to a hundredth of the target so that every
@@ -544,7 +710,7 @@ def main():
     records = {}
     head_events = None
     for form in forms:
-        record, events, _ = run_form(form, ham, system, trial, kwargs, oracle_values, tol_ha,
+        record, events, _ = run_form(form, ham, system, kwargs, oracle_values, tol_ha,
                                      args.steps, args.warmup)
         records[form] = record
         if head_events is None:
@@ -616,20 +782,20 @@ def main():
     }
     if "mixed" in records and "fp64" in records and records["fp64"]["value"] > 0:
         line["config"]["mixed_over_fp64"] = records["mixed"]["value"] / records["fp64"]["value"]
+    line["mfma"] = mfma_leg(ham, system, kwargs, forms)
     if args.breakdown:
         for form in forms:
-            timers = TimingRegistry()
+            timers = _WallTimers()
             torch.cuda.synchronize()
             ham.timers = timers
-            with timers.measure("davidson"):
-                davidson(ham.apply, ham.diagonal, system.nroots, initial_vectors=trial,
-                         operator_into=ham.apply_into, device_output=True, timers=timers,
-                         precision=form, **kwargs)
-                torch.cuda.synchronize()
-            print(f"breakdown ({form}):", file=sys.stderr)
-            for k, v in sorted(timers.entries.items(), key=lambda kv: -kv[1].wall_seconds):
-                print(f"  {k:14s} {v.wall_seconds * 1e3:9.1f} ms  {v.calls:5d} calls",
-                      file=sys.stderr)
+            t0 = time.perf_counter()
+            davidson(ham.apply, ham.diagonal, system.nroots, initial_vectors=system.trial,
+                     operator_into=ham.apply_into, timers=timers, precision=form, **kwargs)
+            torch.cuda.synchronize()
+            print(f"breakdown ({form}): {1e3 * (time.perf_counter() - t0):.1f} ms",
+                  file=sys.stderr)
+            for k, (ms, calls) in sorted(timers.entries.items(), key=lambda kv: -kv[1][0]):
+                print(f"  {k:14s} {ms:9.1f} ms  {calls:5d} calls", file=sys.stderr)
             ham.timers = None
     if not args.no_batch:
         line["batch"] = batch_leg(args, device)
@@ -639,12 +805,14 @@ def main():
         line["density"] = density
         projectors, p_inputs, p_packed = projector_leg(system, device)
         line["projectors"] = projectors
-        iterative, rmm_result = iterative_leg(system, ham, trial)
+        iterative, rmm_result = iterative_leg(system, ham,
+                                              torch.from_numpy(system.trial).to(device))
         line["iterative"] = iterative
         line["wavefunction_io"] = wavefunction_io_leg(system, device, not args.no_cpu)
     if not args.no_cpu:
         happ = head["hamiltonian_applications"][-1]
-        line["cpu_baseline"] = cpu_baseline(system, kwargs, happ, args.cpu_budget)
+        line["cpu_baseline"] = cpu_baseline(system, kwargs, happ, args.cpu_budget,
+                                            args.cpu_repeats)
         if legs:
             line["density"].update(density_cpu_baseline(system, d_coeff, d_weights, d_grid,
                                                         density["ms_per_call"]))

@@ -96,7 +96,9 @@ def _matrix(t: torch.Tensor):
         raise ValueError(_NOT_MATRIX)
     s0, s1 = shape
     st0, st1 = stride
-    if st1 != 1 and s0 > 1 and s1 > 1:
+    if st1 != 1 and s1 > 1:
+        # the kernels read each column as s1 contiguous elements, one column
+        # or many
         raise ValueError(_NOT_MATRIX)
     if t.is_conj() or t.is_neg():
         # a lazy conjugate / negative view: its memory holds the plain values
@@ -128,6 +130,22 @@ class Workspace:
 
 # workspace bytes of a GEMM shape (a pure function of the shape)
 _GEMM_WORKSPACE: dict = {}
+# set to a list to record (class, dtype, m, n, k, start, stop) for every GEMM,
+# with HIP events on the launch stream around it; the class is the label of
+# the enclosing ``gemm_class`` scope of this thread ("other" outside one)
+GEMM_TRACE = None
+
+
+@contextmanager
+def gemm_class(label: str):
+    """Label the GEMMs launched by this thread within the scope (for
+    ``GEMM_TRACE``)."""
+    prev = getattr(_TLS, "gemm_class", None)
+    _TLS.gemm_class = label
+    try:
+        yield
+    finally:
+        _TLS.gemm_class = prev
 
 
 def gemm(op_a: int, m: int, n: int, k: int, alpha, a: torch.Tensor, b: torch.Tensor,
@@ -162,11 +180,19 @@ def gemm(op_a: int, m: int, n: int, k: int, alpha, a: torch.Tensor, b: torch.Ten
             _GEMM_WORKSPACE.clear()
         _GEMM_WORKSPACE[key] = need
     buf = ws.get(need)
+    trace = GEMM_TRACE
+    if trace is not None:
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(torch.cuda.current_stream(c.device))
     rc = lib.qepw_gemm(prec, op_a, m, n, k, nat.complex_pair(alpha), a.data_ptr(), lda,
                        b.data_ptr(), ldb, nat.complex_pair(beta), c.data_ptr(), ldc,
                        buf.data_ptr(), buf.numel(), stream_handle(c.device))
     if rc:
         nat.check(rc, "qepw_gemm")
+    if trace is not None:
+        stop.record(torch.cuda.current_stream(c.device))
+        trace.append((getattr(_TLS, "gemm_class", None) or "other", dt, m, n, k, start, stop))
 
 
 def column_squared_norms(v: torch.Tensor, nrows: int, out: torch.Tensor | None = None) -> torch.Tensor:

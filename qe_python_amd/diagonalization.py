@@ -152,6 +152,9 @@ def _lowest_generalized_eigh(hamiltonian, overlap, roots):
 
 def _measure(timers, label, device=None):
     if timers is None:
+        if dv.GEMM_TRACE is not None:
+            # GEMMs traced by class: the timer label names the class
+            return dv.gemm_class(label)
         return nullcontext()
     return _SyncedMeasure(timers, label, device)
 
@@ -652,7 +655,9 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
       ``tolerance``.  ``None`` selects "mixed" when the operator is a
       ``PlaneWaveHamiltonian`` built with ``precision="mixed"``, else "fp64".
       The result's ``iterations`` and ``hamiltonian_applications`` count
-      both phases.
+      both phases, which share ``max_iterations`` (the complex64 phase
+      takes at most ``max_iterations - 1``, the refinement the rest; with
+      ``max_iterations < 2`` the FP64 form runs).
     """
     if reduced_eigensolver is None:
         reduced_eigensolver = REDUCED_EIGENSOLVER
@@ -732,6 +737,9 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
             raise ValueError("initial_applied must match the initial Ritz block")
         return initial_applied
 
+    if precision == "mixed" and max_iterations < 2:
+        # no room for a complex64 iteration and the FP64 refinement under the cap
+        precision = "fp64"
     if precision == "fp64":
         dtype = torch.complex128
         start = _upload_block(trial, perm_dev, npw, device, dtype)
@@ -743,13 +751,16 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
                               phase.residual_norms, phase.iterations, phase.converged,
                               phase.number_unconverged, phase.happ)
 
-    # mixed: complex64 phase to the switch thresholds ...
+    # mixed: complex64 phase to the switch thresholds ...  Both phases share
+    # max_iterations (the SCF sums ``iterations`` over its retries,
+    # pw/scf.py:3539-3556): the complex64 phase may use all but one, the
+    # FP64 refinement what is left
     de_switch, res_switch = switch
     start = _upload_block(trial, perm_dev, npw, device, torch.complex64)
     coarse = _davidson_phase(
         ctx, start, torch.complex64, np.maximum(energy_bounds, de_switch),
         np.full(number_of_roots, res_switch), None, np.maximum(root_tolerances, de_switch),
-        max_iterations, initial_is_ritz, initial_is_orthonormal, initial_eigenvalues,
+        max_iterations - 1, initial_is_ritz, initial_is_orthonormal, initial_eigenvalues,
         applied_block, perm_dev, stall_window=MIXED_STALL_WINDOW, require_residuals=True,
         coarse=True)
     del start
@@ -759,8 +770,8 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
     coarse.X = None
     fine = _davidson_phase(
         ctx, start, torch.complex128, energy_bounds, residual_bounds, residual_energy_scale,
-        root_tolerances, max_iterations, False, True, None, lambda active: None, perm_dev,
-        seed_spectrum=coarse.spectrum)
+        root_tolerances, max(1, max_iterations - coarse.iterations), False, True, None,
+        lambda active: None, perm_dev, seed_spectrum=coarse.spectrum)
     return DavidsonResult(fine.values, _result_vectors(fine.X, perm_dev, npw, device_output),
                           fine.residual_norms, coarse.iterations + fine.iterations,
                           fine.converged, fine.number_unconverged, coarse.happ + fine.happ)

@@ -84,6 +84,12 @@ def test_gemm_wrapper_guards_raise_before_the_library():
         dv.gemm(nat.OP_N, 4, 4, 4, 1.0, z(2, 4, 4), z(4, 4), 0.0, z(4, 4), None)
     with pytest.raises(ValueError, match="unit inner stride"):
         dv.gemm(nat.OP_N, 4, 4, 4, 1.0, z(4, 8)[:, ::2], z(4, 4), 0.0, z(4, 4), None)
+    # one-column operands with strided elements: the kernels would read the
+    # column contiguously
+    with pytest.raises(ValueError, match="unit inner stride"):
+        dv.gemm(nat.OP_N, 4, 1, 4, 1.0, z(4, 4), z(1, 8)[:, ::2], 0.0, z(1, 4), None)
+    with pytest.raises(ValueError, match="unit inner stride"):
+        dv.gemm(nat.OP_C, 1, 1, 4, 1.0, z(4, 4).T[:1], z(1, 4), 0.0, z(1, 1), None)
     with pytest.raises(ValueError, match="lazy conj"):
         dv.gemm(nat.OP_N, 4, 4, 4, 1.0, z(4, 4).conj(), z(4, 4), 0.0, z(4, 4), None)
     with pytest.raises(ValueError, match="one complex dtype"):

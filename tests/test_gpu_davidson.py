@@ -20,6 +20,19 @@ GOLDEN = Path(__file__).parent / "golden"
 TOL_HA = 0.5e-10
 
 
+class LabelRecorder:
+    """The ``measure(label)`` interface of the SCF's timer registry
+    (``qepy_pw/timing.py:38-63``), recording the labels used."""
+
+    def __init__(self):
+        self.labels = set()
+
+    def measure(self, label):
+        from contextlib import nullcontext
+        self.labels.add(label)
+        return nullcontext()
+
+
 def device_ham(s, **kw):
     from qe_python_amd.diagonalization import PackedProjectorTerm, PlaneWaveHamiltonian
     beta, coup, diag = s.packed_term()
@@ -79,6 +92,48 @@ def test_matches_oracle_trajectory_closely(small):
     assert abs(res.iterations - ref.iterations) <= 3
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_config1_eight_roots_reference_stop_test(small, precision):
+    """Config 1 as written: 8 roots from 16 trials at 1e-10 Ry with the
+    reference's default stop test (|de| < tol and residual < 10 tol,
+    diagonalization.py:811-828) and 40 iterations.  Root 8 (0.37239 Ha) sits
+    1.1e-3 Ha under a near-degenerate pair, and the oracle Davidson does not
+    converge it in 40 iterations: the device must reach the same outcome --
+    not converged, roots 1-7 at the dense eigenvalues, root 8 still an upper
+    bound of the 8th eigenvalue.  The mixed form (config 3) shares the cap and
+    is held to the 1e-8 Ry target on roots 1-7."""
+    from qe_python_amd.diagonalization import davidson
+    s, ham, exact = small
+    kwargs = dict(initial_vectors=s.trial, tolerance=TOL_HA, max_iterations=40)
+    from types import SimpleNamespace
+    bs = s.packed_term()
+    oh = OracleHamiltonian(s.basis.indices, s.basis.kinetic, s.shape, s.real_potential,
+                           (SimpleNamespace(beta=bs[0], coupling=bs[1], diagonal=bs[2]),),
+                           potential_average=float(np.real(s.potential_g.flat[0])))
+    ref = odav.davidson(oh.apply, oh.diagonal, 8, operator_into=oh.apply_into, **kwargs)
+    res = davidson(ham.apply, ham.diagonal, 8, operator_into=ham.apply_into,
+                   precision=precision, **kwargs)
+    err, ref_err = res.eigenvalues - exact[:8], ref.eigenvalues - exact[:8]
+    print(f"{precision}: device converged={res.converged} unconverged={res.number_unconverged} "
+          f"iterations={res.iterations}; oracle converged={ref.converged} "
+          f"unconverged={ref.number_unconverged} iterations={ref.iterations}; "
+          f"errors device " + " ".join(f"{e:.1e}" for e in err)
+          + " oracle " + " ".join(f"{e:.1e}" for e in ref_err))
+    assert not ref.converged and ref.number_unconverged == 1 and ref.iterations == 40
+    assert not res.converged and res.iterations <= 40
+    bar = TOL_HA if precision == "fp64" else 0.5e-8
+    assert np.all(np.abs(err[:7]) < bar), err
+    assert np.all(np.abs(ref_err[:7]) < TOL_HA), ref_err
+    # the stalled root: an upper bound (Ritz value) still far from converged
+    # on both sides, as the stop test reports
+    assert -1e-12 < err[7] < 1e-3 and -1e-12 < ref_err[7] < 1e-3
+    if precision == "fp64":
+        assert res.number_unconverged == ref.number_unconverged
+        assert res.residual_norms[7] > 10 * TOL_HA
+    else:
+        assert res.number_unconverged >= 1
+
+
 SCF_OPTIONS = [
     # the SCF's keywords (pw/scf.py:3510-3537): occupied roots with the looser
     # test for the empty ones (diagonalization.py:581-583), the
@@ -97,10 +152,9 @@ def test_scf_options_match_oracle(small, opts):
     from types import SimpleNamespace
 
     from qe_python_amd.diagonalization import davidson
-    from qe_python_amd.timing import TimingRegistry
     s, ham, exact = small
     nroots = 7
-    timers = TimingRegistry()
+    timers = LabelRecorder()
     start = s.trial[:, :nroots] if "subspace_multiplier" in opts else s.trial
     res = davidson(ham.apply, ham.diagonal, nroots, initial_vectors=start, tolerance=TOL_HA,
                    operator_into=ham.apply_into, timers=timers, **opts)
@@ -122,7 +176,9 @@ def test_scf_options_match_oracle(small, opts):
     np.testing.assert_allclose(res.eigenvalues[:occ], ref.eigenvalues[:occ],
                                atol=10 * TOL_HA, rtol=0)
     assert abs(res.iterations - ref.iterations) <= 3
-    assert timers.entries, "timer labels were not recorded"
+    # the reference's QE timer labels (timing.py:38-63 interface: measure(label))
+    assert {"cegterg:over", "cegterg:upda", "cdiaghg", "g_psi"} \
+        <= set(timers.labels), timers.labels
 
 
 def test_generic_operator_reference_pattern():

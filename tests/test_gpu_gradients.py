@@ -124,6 +124,54 @@ def test_medium_workload_backward_against_oracle():
     assert rel(dv32, dv) < 2e-5 and rel(dtau32, dtau) < 2e-5 and rel(gpsi32, gpsi) < 2e-5
 
 
+def test_medium_workload_central_differences():
+    """The differentiation workload at its own size (72^3 grid, 64 atoms,
+    128 bands, f = 2) against central differences of the FP64 device energy
+    with the config's steps: h = 1e-5 bohr in tau (the three largest force
+    components) and 1e-6 Ry in V(r) (E is linear in V, so that difference is
+    exact up to rounding; it is taken along a seeded direction over the whole
+    grid).  The complex128 gradient is held to 1e-6 and the complex64 one to
+    1e-4 of each difference."""
+    from qe_python_amd.gradients import BandEnergyModel, band_energy
+    s = make_system("medium")
+    nb = 128
+    psi = s.trial[:, :nb]
+    f = np.full(nb, 2.0)
+    model = BandEnergyModel(s.basis, s.shape, s.projectors.beta_species, s.projectors.couplings)
+    _, dv64, dtau64, _ = run(model, s, psi, f)
+    _, dv32, dtau32, _ = run(model, s, psi, f, precision="fp32")
+    p_dev = torch.tensor(psi, device=DEV)
+    v0 = s.real_potential
+
+    def energy(positions=None, potential=None):
+        with torch.no_grad():
+            return float(band_energy(
+                model, p_dev, torch.tensor(v0 if potential is None else potential, device=DEV),
+                torch.tensor(s.projectors.positions if positions is None else positions,
+                             device=DEV), f))
+
+    h = 1.0e-5
+    flat = np.argsort(np.abs(dtau64).ravel())[::-1][:3]
+    for a, c in zip(*np.unravel_index(flat, dtau64.shape)):
+        plus = s.projectors.positions.copy()
+        minus = s.projectors.positions.copy()
+        plus[a, c] += h
+        minus[a, c] -= h
+        fd = (energy(positions=plus) - energy(positions=minus)) / (2 * h)
+        print(f"tau[{a},{c}]: fd {fd:.10e} c128 {dtau64[a, c]:.10e} c64 {dtau32[a, c]:.10e}")
+        assert dtau64[a, c] == pytest.approx(fd, rel=1e-6)
+        assert dtau32[a, c] == pytest.approx(fd, rel=1e-4)
+    # V(r): a step of 1e-6 Ry along a seeded direction over the whole grid
+    # (a single grid point would move E by less than its FP64 rounding)
+    hv = 0.5e-6
+    d = np.random.default_rng(11).standard_normal(v0.shape)
+    fd = (energy(potential=v0 + hv * d) - energy(potential=v0 - hv * d)) / (2 * hv)
+    g64, g32 = float(np.sum(dv64 * d)), float(np.sum(dv32 * d))
+    print(f"V along d: fd {fd:.10e} c128 {g64:.10e} c64 {g32:.10e}")
+    assert g64 == pytest.approx(fd, rel=1e-6)
+    assert g32 == pytest.approx(fd, rel=1e-4)
+
+
 def test_argument_errors(small):
     from qe_python_amd.gradients import band_energy
     s, psi, f, model, _ = small
