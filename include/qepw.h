@@ -129,6 +129,27 @@ int qepw_gemm(int precision, int op_a, int64_t m, int64_t n, int64_t k,
               size_t ws_bytes, void* stream);
 
 /*
+ * qepw_gemm without a workspace argument and without any allocation: the
+ * split-K partial tiles go to a static device array of
+ * QEPW_GEMM_STATIC_SCRATCH bytes (32 MiB) that each GPU the library runs on
+ * holds until the process ends (qepw_gemm_static_scratch_bytes() returns the
+ * size).  Same arguments, kernels and results as qepw_gemm (bitwise: the
+ * same split plan and fixed-order reduction).  A product whose split-K
+ * partials would exceed the array returns QEPW_ERR_WORKSPACE with a message
+ * naming the size.  Calls that overlap on one GPU share its array and race;
+ * the array needs no clearing between calls (every partial tile is written
+ * before it is summed).  Call with the stream's device current.
+ * No counterpart in the reference: this entry point is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).
+ */
+#define QEPW_GEMM_STATIC_SCRATCH ((size_t)32 << 20)
+size_t qepw_gemm_static_scratch_bytes(void);
+int qepw_gemm_static(int precision, int op_a, int64_t m, int64_t n, int64_t k,
+                     const double* alpha, const void* a, int64_t lda, const void* b,
+                     int64_t ldb, const double* beta, void* c, int64_t ldc, void* stream);
+
+/*
  * out[b] = sum_g |v[g, b]|^2 (device double, FP64 accumulation for both
  * precisions, fixed reduction order).
  * Replaces: column_squared_norms (qepy_pw/_native_fft.pyx:1153-1191).

@@ -46,6 +46,10 @@ SIGNATURES = [
     ("qepw_gemm", c_int, [c_int, c_int, c_int64, c_int64, c_int64, POINTER(c_double), c_void_p,
                           c_int64, c_void_p, c_int64, POINTER(c_double), c_void_p, c_int64,
                           c_void_p, c_size_t, c_void_p]),
+    ("qepw_gemm_static_scratch_bytes", c_size_t, []),
+    ("qepw_gemm_static", c_int, [c_int, c_int, c_int64, c_int64, c_int64, POINTER(c_double),
+                                 c_void_p, c_int64, c_void_p, c_int64, POINTER(c_double),
+                                 c_void_p, c_int64, c_void_p]),
     ("qepw_column_squared_norms", c_int, [c_int, c_void_p, c_int64, c_int64, c_int, c_void_p,
                                           c_void_p]),
     ("qepw_column_inner_products", c_int, [c_int, c_void_p, c_int64, c_void_p, c_int64, c_int64,

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -361,6 +362,75 @@ int qepw_gemm(int precision, int op_a, int64_t m, int64_t n, int64_t k, const do
   return finish(qepw::gemm(precision, op_a, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
                            workspace, ws_bytes, static_cast<hipStream_t>(stream)),
                 "qepw_gemm");
+}
+
+}  // extern "C"
+
+// Scratch of qepw_gemm_static: the split-K partial tiles live in this
+// file-scope device array instead of a caller or library allocation.  Each
+// GPU the library runs on holds its own copy (QEPW_GEMM_STATIC_SCRATCH
+// bytes) until the process ends.  Host code never uses the array's name as an
+// address: the device address of each GPU's copy comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+namespace {
+constexpr size_t kStaticScratchBytes = QEPW_GEMM_STATIC_SCRATCH;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the calling thread's current device's scratch (or an
+// error code from the lookup).
+int static_scratch(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail_hip(e, "qepw_gemm_static: hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(QEPW_ERR_UNSUPPORTED, "qepw_gemm_static: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail_hip(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                      "qepw_gemm_static: hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return QEPW_OK;
+}
+}  // namespace
+
+extern "C" {
+
+size_t qepw_gemm_static_scratch_bytes(void) { return kStaticScratchBytes; }
+
+int qepw_gemm_static(int precision, int op_a, int64_t m, int64_t n, int64_t k,
+                     const double* alpha, const void* a, int64_t lda, const void* b,
+                     int64_t ldb, const double* beta, void* c, int64_t ldc, void* stream) {
+  if (!alpha || !beta) return fail(QEPW_ERR_ARG, "qepw_gemm_static: NULL alpha/beta");
+  if ((m > 0 && n > 0) && (!c || (k > 0 && (!a || !b))))
+    return fail(QEPW_ERR_ARG, "qepw_gemm_static: NULL matrix");
+  if (m < 0 || n < 0 || k < 0) return fail(QEPW_ERR_ARG, "qepw_gemm_static: negative size");
+  if (precision != QEPW_C128 && precision != QEPW_C64)
+    return fail(QEPW_ERR_ARG, "qepw_gemm_static: unknown precision");
+  const size_t need = qepw::gemm_workspace_bytes(precision, op_a, m, n, k);
+  if (need > kStaticScratchBytes) {
+    char msg[192];
+    std::snprintf(msg, sizeof msg,
+                  "qepw_gemm_static: this product needs %zu bytes of split-K scratch, more "
+                  "than the static %zu (use qepw_gemm with a workspace)",
+                  need, kStaticScratchBytes);
+    return fail(QEPW_ERR_WORKSPACE, msg);
+  }
+  void* ptr = nullptr;
+  if (need > 0) {
+    const int rc = static_scratch(&ptr);
+    if (rc != QEPW_OK) return rc;
+  }
+  return finish(qepw::gemm(precision, op_a, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ptr,
+                           need > 0 ? kStaticScratchBytes : 0, static_cast<hipStream_t>(stream)),
+                "qepw_gemm_static");
 }
 
 int qepw_column_squared_norms(int precision, const void* v, int64_t ldv, int64_t nrows, int nb,
