@@ -384,8 +384,10 @@ def batch_leg(args, device):
 
     golden = np.load(ROOT / "tests" / "golden" / "batch_davidson.npz")
     tol_ha = 0.5 * args.tol_ry
+    # some problems need more than 40 iterations at this tolerance (both
+    # phases of the mixed form count against the cap)
     kwargs = dict(tolerance=args.tol_factor * tol_ha, residual_factor=None,
-                  subspace_multiplier=4, max_iterations=40, precision="mixed")
+                  subspace_multiplier=4, max_iterations=100, precision="mixed")
     problems, names, npws, roots = [], [], [], []
     for cell in ("cell54", "cell64", "cell72"):
         for k, kp in enumerate(batch_kpoints(cell)):

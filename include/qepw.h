@@ -129,6 +129,24 @@ int qepw_gemm(int precision, int op_a, int64_t m, int64_t n, int64_t k,
               size_t ws_bytes, void* stream);
 
 /*
+ * Tile choice of qepw_gemm (the autotuner's interface; no counterpart in the
+ * reference, whose BLAS picks its own blocking).  The kernel has
+ * qepw_gemm_variant_count() tile variants (TM x TN output tiles per
+ * workgroup); qepw_gemm_plan writes {variant, K splits, TM, TN} of a shape
+ * to out[0..3].  qepw_gemm_set_rule adds a rule -- ranges[0..5] = m_lo,
+ * m_hi, n_lo, n_hi, k_lo, k_hi (inclusive) for one precision and op_a --
+ * that overrides the built-in choice (first matching rule wins; at most 256
+ * rules); qepw_gemm_clear_rules drops them.  Rules apply to qepw_gemm,
+ * qepw_gemm_workspace and qepw_gemm_static alike: set them before products
+ * run, not while another thread launches GEMMs.
+ */
+int qepw_gemm_variant_count(void);
+int qepw_gemm_plan(int precision, int op_a, int64_t m, int64_t n, int64_t k, int64_t* out);
+int qepw_gemm_set_rule(int precision, int op_a, const int64_t* ranges, int variant,
+                       int64_t splits);
+int qepw_gemm_clear_rules(void);
+
+/*
  * qepw_gemm without a workspace argument and without any allocation: the
  * split-K partial tiles go to a static device array of
  * QEPW_GEMM_STATIC_SCRATCH bytes (32 MiB) that each GPU the library runs on
@@ -214,6 +232,27 @@ int qepw_restart_applied(int precision, const void* x, int64_t ldx, const void* 
 int qepw_permute_rows(int src_precision, int dst_precision, int direction,
                       const int32_t* perm, const void* src, int64_t lds, void* dst,
                       int64_t ldd, int64_t nrows, int nb, void* stream);
+
+/*
+ * dst[:, dst_cols[j]] = scale[j] * src[:, src_cols[j]] for j < ncols, nrows
+ * rows each (src_cols, dst_cols: device int32, NULL for j; scale: device
+ * double, NULL for 1), widening or narrowing between the precisions.  The
+ * column selections, merges and conversions of davidson's bookkeeping: the
+ * Ritz coefficients of the unconverged roots and C diag(e)
+ * (qepy_pw/diagonalization.py:751-757, ritz_pairs 699-726), the restart and
+ * locking blocks (843-868).
+ */
+int qepw_copy_columns(int src_precision, int dst_precision, const void* src, int64_t lds,
+                      const int32_t* src_cols, void* dst, int64_t ldd, const int32_t* dst_cols,
+                      const double* scale, int64_t nrows, int ncols, void* stream);
+/*
+ * dst[i, j] = (i == j) ? (d ? d[i] : diag) : 0 over an nrows x ncols block
+ * (zero, identity or a real diagonal; d: device double or NULL): the restart
+ * block's H = diag(e), S = I and identity coefficients (diagonalization.py:
+ * 843-868), and zeroed storage.
+ */
+int qepw_set_block(int precision, void* dst, int64_t ldd, int64_t nrows, int ncols, double diag,
+                   const double* d, void* stream);
 
 /*
  * diagonal[g] += multiplicity * Re sum_i coupled[g, i] conj(beta[g, i])

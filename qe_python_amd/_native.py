@@ -10,6 +10,7 @@ invalid arguments, ``RuntimeError`` for HIP / unsupported-shape failures.
 from __future__ import annotations
 
 import ctypes
+import os
 from ctypes import c_double, c_int, c_int64, c_size_t, c_void_p, POINTER
 from pathlib import Path
 
@@ -46,6 +47,10 @@ SIGNATURES = [
     ("qepw_gemm", c_int, [c_int, c_int, c_int64, c_int64, c_int64, POINTER(c_double), c_void_p,
                           c_int64, c_void_p, c_int64, POINTER(c_double), c_void_p, c_int64,
                           c_void_p, c_size_t, c_void_p]),
+    ("qepw_gemm_variant_count", c_int, []),
+    ("qepw_gemm_plan", c_int, [c_int, c_int, c_int64, c_int64, c_int64, POINTER(c_int64)]),
+    ("qepw_gemm_set_rule", c_int, [c_int, c_int, POINTER(c_int64), c_int, c_int64]),
+    ("qepw_gemm_clear_rules", c_int, []),
     ("qepw_gemm_static_scratch_bytes", c_size_t, []),
     ("qepw_gemm_static", c_int, [c_int, c_int, c_int64, c_int64, c_int64, POINTER(c_double),
                                  c_void_p, c_int64, c_void_p, c_int64, POINTER(c_double),
@@ -64,6 +69,10 @@ SIGNATURES = [
                                      c_void_p, c_int64, c_int64, c_int, c_void_p]),
     ("qepw_permute_rows", c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p,
                                   c_int64, c_int64, c_int, c_void_p]),
+    ("qepw_copy_columns", c_int, [c_int, c_int, c_void_p, c_int64, c_void_p, c_void_p, c_int64,
+                                  c_void_p, c_void_p, c_int64, c_int, c_void_p]),
+    ("qepw_set_block", c_int, [c_int, c_void_p, c_int64, c_int64, c_int, c_double, c_void_p,
+                               c_void_p]),
     ("qepw_accumulate_projector_diagonal", c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int64,
                                                    c_int, c_double, c_void_p, c_void_p]),
     ("qepw_accumulate_density_workspace", c_size_t, [c_void_p, c_int, c_int]),
@@ -122,6 +131,12 @@ def load() -> ctypes.CDLL:
         fn.restype = restype
         fn.argtypes = argtypes
     _LIB = lib
+    # the committed GEMM tile table (qe_python_amd.autotune); QEPW_GEMM_TABLE
+    # names another table, or "none" for the built-in choice
+    table = os.environ.get("QEPW_GEMM_TABLE")
+    if table != "none":
+        from . import autotune
+        autotune.load_table(table)
     return lib
 
 

@@ -364,6 +364,25 @@ int qepw_gemm(int precision, int op_a, int64_t m, int64_t n, int64_t k, const do
                 "qepw_gemm");
 }
 
+int qepw_gemm_variant_count(void) { return qepw::gemm_variant_count(); }
+
+int qepw_gemm_plan(int precision, int op_a, int64_t m, int64_t n, int64_t k, int64_t* out) {
+  if (!out || m < 0 || n < 0 || k < 0) return fail(QEPW_ERR_ARG, "qepw_gemm_plan: bad argument");
+  return finish(qepw::gemm_plan(precision, op_a, m, n, k, out), "qepw_gemm_plan");
+}
+
+int qepw_gemm_set_rule(int precision, int op_a, const int64_t* ranges, int variant,
+                       int64_t splits) {
+  const int rc = qepw::gemm_set_rule(precision, op_a, ranges, variant, splits);
+  if (rc == QEPW_ERR_UNSUPPORTED) return fail(rc, "qepw_gemm_set_rule: rule table full");
+  return finish(rc, "qepw_gemm_set_rule");
+}
+
+int qepw_gemm_clear_rules(void) {
+  qepw::gemm_clear_rules();
+  return QEPW_OK;
+}
+
 }  // extern "C"
 
 // Scratch of qepw_gemm_static: the split-K partial tiles live in this
@@ -501,6 +520,28 @@ int qepw_permute_rows(int src_precision, int dst_precision, int direction, const
   return finish(qepw::permute_rows(src_precision, dst_precision, direction, perm, src, lds, dst,
                                    ldd, nrows, nb, static_cast<hipStream_t>(stream)),
                 "qepw_permute_rows");
+}
+
+int qepw_copy_columns(int src_precision, int dst_precision, const void* src, int64_t lds,
+                      const int32_t* src_cols, void* dst, int64_t ldd, const int32_t* dst_cols,
+                      const double* scale, int64_t nrows, int ncols, void* stream) {
+  if (ncols > 0 && nrows > 0 && (!src || !dst || lds < nrows || ldd < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_copy_columns: bad argument");
+  if (ncols < 0 || nrows < 0) return fail(QEPW_ERR_ARG, "qepw_copy_columns: negative size");
+  return finish(qepw::copy_columns(src_precision, dst_precision, src, lds, src_cols, dst, ldd,
+                                   dst_cols, scale, nrows, ncols,
+                                   static_cast<hipStream_t>(stream)),
+                "qepw_copy_columns");
+}
+
+int qepw_set_block(int precision, void* dst, int64_t ldd, int64_t nrows, int ncols, double diag,
+                   const double* d, void* stream) {
+  if (ncols > 0 && nrows > 0 && (!dst || ldd < nrows))
+    return fail(QEPW_ERR_ARG, "qepw_set_block: bad argument");
+  if (ncols < 0 || nrows < 0) return fail(QEPW_ERR_ARG, "qepw_set_block: negative size");
+  return finish(qepw::set_block(precision, dst, ldd, nrows, ncols, diag, d,
+                                static_cast<hipStream_t>(stream)),
+                "qepw_set_block");
 }
 
 int qepw_accumulate_projector_diagonal(const void* coupled, int64_t ldc, const void* beta,

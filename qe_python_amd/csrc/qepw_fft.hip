@@ -966,6 +966,62 @@ __global__ void k_sum_slices(const double* __restrict__ partial, int slices, int
   out[i] = scale * s;
 }
 
+// Density rows of the split stage: a workgroup owns rows y0..y0+nr of plane
+// z and, for every band of its slice, gathers them from the column lines of
+// k_split_cols_inverse, runs the backward x-FFT and adds w_b |psi(r)|^2 to
+// its register accumulators; the sum over its bands is written (or added)
+// once, so the result does not depend on how bands are scheduled.
+template <typename T>
+__global__ __launch_bounds__(kSplitThreads) void k_split_rows_density(
+    const cplx<T>* __restrict__ tbuf, int64_t tband, const int* __restrict__ xcols, int nxc,
+    int n1, int n2, int n3, int nb, int bands_per_slice, const double* __restrict__ weights,
+    double* __restrict__ partial, int accumulate, int group, LinePlan p1,
+    const cplx<T>* __restrict__ tw1) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* Ls = reinterpret_cast<cplx<T>*>(smem_raw);
+  constexpr int kPer = kPlanePoints;  // row points owned per thread (nr n1 <= NT * kPer)
+  const int tid = threadIdx.x;
+  const int y0 = blockIdx.x * group;
+  const int nr = min(group, n2 - y0);
+  const int z = blockIdx.y;
+  const int slice = blockIdx.z;
+  const int b0 = slice * bands_per_slice;
+  const int b1 = min(nb, b0 + bands_per_slice);
+  const int npts = nr * n1;
+  const cplx<T> zero = mk<T>(T(0), T(0));
+  double acc[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) acc[k] = 0.0;
+  for (int b = b0; b < b1; ++b) {
+    const double wb = weights[b];
+    for (int i = tid; i < npts; i += kSplitThreads) Ls[i] = zero;
+    __syncthreads();
+    const cplx<T>* base = tbuf + (int64_t)b * tband + (int64_t)z * nxc * n2 + y0;
+    for (int i = tid; i < nr * nxc; i += kSplitThreads) {
+      const int r = i % nr, c = i / nr;
+      Ls[r * n1 + xcols[c]] = base[(int64_t)c * n2 + r];
+    }
+    __syncthreads();
+    lds_fft<T, kSplitThreads, kPlanePoints>(Ls, nr, n1, 1, nullptr, p1, tw1, +1, tid);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = tid + k * kSplitThreads;
+      if (i < npts) {
+        const cplx<T> v = Ls[i];
+        acc[k] += wb * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+      }
+    }
+    __syncthreads();
+  }
+  // partial[slice] is an (n3, n2, n1) grid; Ls index r * n1 + x is row y0 + r
+  double* out = partial + ((int64_t)slice * n3 + z) * ((int64_t)n1 * n2) + (int64_t)y0 * n1;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = tid + k * kSplitThreads;
+    if (i < npts) out[i] = (accumulate ? out[i] : 0.0) + acc[k];
+  }
+}
+
 constexpr int kDensitySlices = 16;
 
 size_t density_workspace_bytes(const Plan& p, int precision, int nb) {
@@ -991,23 +1047,40 @@ int accumulate_density_t(const Plan& p, const void* vectors, int64_t ldv, int nb
     return QEPW_ERR_UNSUPPORTED;
   const size_t team_lds = (p.n1 == p.n2 && st.d_stickq) ? density_team_lds<T>(p.n1, st.nxcols) : 0;
   const bool team = team_lds > 0 && team_lds <= (size_t)kMaxLdsBytes;
-  if (!team) {
+  // planes that neither the team kernel nor one workgroup's LDS can hold go
+  // through the split stage's column lines, as in apply_local
+  const bool split = !team && !generic_plane_fits<T>(p) && split_available<T>(p, st);
+  if (!team && !split) {
     int rc = check_fft_limits<T>(p, st);
     if (rc) return rc;
   }
   const size_t partial_bytes = (size_t)kDensitySlices * ngrid * sizeof(double);
-  const size_t band_bytes = (size_t)p.n3 * st.nst * sizeof(cplx<T>);
+  const size_t stick_band = (size_t)p.n3 * st.nst * sizeof(cplx<T>);
+  const int64_t tband = split ? split_band_elems(p, st) : 0;
+  const size_t band_bytes = stick_band + (size_t)tband * sizeof(cplx<T>);
   if (ws_bytes < partial_bytes + band_bytes) return QEPW_ERR_WORKSPACE;
   double* partial = static_cast<double*>(workspace);
   cplx<T>* work = reinterpret_cast<cplx<T>*>(static_cast<char*>(workspace) + partial_bytes);
   int chunk = (int)((ws_bytes - partial_bytes) / band_bytes);
   if (chunk > 65535) chunk = 65535;
+  // the split stage's column lines follow the chunk's stick buffers
+  cplx<T>* tbuf = work + (int64_t)chunk * p.n3 * st.nst;
   const size_t stick_lds = (size_t)st.group * (p.n3 + 1) * sizeof(cplx<T>);
   const size_t plane_lds = np * sizeof(cplx<T>);
   set_lds_attrs<T>(stick_lds, 0);
-  if (!team)
+  if (!team && !split)
     hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density_plane<T>),
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_lds);
+  const int gc = split ? split_group<T>(p.line[1]) : 0;
+  const int gr = split ? split_group<T>(p.line[0]) : 0;
+  const size_t lc = (size_t)gc * p.n2 * sizeof(cplx<T>);
+  const size_t lr = (size_t)gr * p.n1 * sizeof(cplx<T>);
+  if (split) {
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_split_cols_inverse<T>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lc);
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_split_rows_density<T>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr);
+  }
   const int64_t bstride = (int64_t)p.n3 * st.nst;
   if (nb <= 0) {
     hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((ngrid + 255) / 256)), dim3(256), 0,
@@ -1044,6 +1117,17 @@ int accumulate_density_t(const Plan& p, const void* vectors, int64_t ldv, int nb
       QEPW_TEAM_SIZES(QEPW_LAUNCH_DENSITY)
 #undef QEPW_LAUNCH_DENSITY
 #undef QEPW_LAUNCH_DENSITY_K
+    }
+    if (!launched && split) {
+      const dim3 gcols((unsigned)((st.nxcols + gc - 1) / gc), (unsigned)p.n3, (unsigned)nbc);
+      hipLaunchKernelGGL(k_split_cols_inverse<T>, gcols, dim3(kSplitThreads), lc, stream, work,
+                         bstride, st.nst, st.d_colstart, st.d_stickq, st.nxcols, p.n2, tbuf,
+                         tband, gc, p.line[1], twiddles<T>(p, 1));
+      const dim3 grows((unsigned)((p.n2 + gr - 1) / gr), (unsigned)p.n3, kDensitySlices);
+      hipLaunchKernelGGL(k_split_rows_density<T>, grows, dim3(kSplitThreads), lr, stream, tbuf,
+                         tband, st.d_xcols, st.nxcols, p.n1, p.n2, p.n3, nbc, per, weights + b0,
+                         partial, acc, gr, p.line[0], twiddles<T>(p, 0));
+      launched = true;
     }
     if (!launched)
       hipLaunchKernelGGL(k_density_plane<T>, grid, dim3(kPlaneThreads), plane_lds, stream, work,

@@ -19,6 +19,8 @@
 // k l/16); B element (k l/16, col l%16).  Accumulator v of lane l holds
 // column l%16 and row 4*(l/16)+v for the f32 instruction but row l/16+4v
 // for the f64 one (measured on gfx950 with an in-bounds probe).
+#include <type_traits>
+
 #include "qepw_internal.h"
 
 namespace qepw {
@@ -42,57 +44,67 @@ template <> struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-constexpr int kTile = 64;     // largest tile (grid-size checks)
 constexpr int kBK = 16;
 constexpr int kGemmThreads = 256;
 constexpr int kPad = 1;
 
-template <typename T, bool CONJ_A, bool CONJ_B, int TILE>
+// TM x TN output tile per 256-thread workgroup; its four waves form a
+// WAVES_M x (4 / WAVES_M) grid, each owning a (TM / WAVES_M) x (TN * WAVES_M / 4)
+// block of 16x16 MFMA tiles.  Rectangular tiles let a tall-skinny product
+// (npw x 128 outputs) cover all its columns in one workgroup, so the tall
+// operand is streamed from HBM once.
+template <typename T, bool CONJ_A, bool CONJ_B, int TM, int TN, int WAVES_M>
 __global__ __launch_bounds__(kGemmThreads) void k_cgemm(
     int64_t M, int64_t N, int64_t K, int64_t kchunk, const cplx<T>* __restrict__ A,
     int64_t lda, const cplx<T>* __restrict__ B, int64_t ldb, cplx<T>* __restrict__ C,
     int64_t ldc, cplx<T>* __restrict__ partial, T alpha_r, T alpha_i, T beta_r,
     T beta_i) {
   using Acc = typename Mfma<T>::acc;
-  constexpr int WT = TILE / 2;    // wave tile
-  constexpr int NB = WT / 16;     // MFMA blocks per wave and dimension
-  __shared__ cplx<T> As[kBK][TILE + kPad];
-  __shared__ cplx<T> Bs[kBK][TILE + kPad];
+  constexpr int WAVES_N = 4 / WAVES_M;
+  constexpr int WTM = TM / WAVES_M;  // wave tile
+  constexpr int WTN = TN / WAVES_N;
+  constexpr int NBM = WTM / 16;      // MFMA blocks per wave and dimension
+  constexpr int NBN = WTN / 16;
+  static_assert(NBM >= 1 && NBN >= 1 && WAVES_M * WAVES_N == 4, "tile shape");
+  __shared__ cplx<T> As[kBK][TM + kPad];
+  __shared__ cplx<T> Bs[kBK][TN + kPad];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = (wave & 1) * WT;
-  const int wn = (wave >> 1) * WT;
-  const int64_t m0 = (int64_t)blockIdx.x * TILE;
-  const int64_t n0 = (int64_t)blockIdx.y * TILE;
+  const int wm = (wave % WAVES_M) * WTM;
+  const int wn = (wave / WAVES_M) * WTN;
+  const int64_t m0 = (int64_t)blockIdx.x * TM;
+  const int64_t n0 = (int64_t)blockIdx.y * TN;
   const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
   const int64_t kend = (kbeg + kchunk) < K ? (kbeg + kchunk) : K;
   const cplx<T> zero = mk<T>(T(0), T(0));
 
-  Acc re[NB][NB], im[NB][NB];
+  Acc re[NBM][NBN], im[NBM][NBN];
 #pragma unroll
-  for (int i = 0; i < NB; ++i)
+  for (int i = 0; i < NBM; ++i)
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
+    for (int j = 0; j < NBN; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) { re[i][j][v] = T(0); im[i][j][v] = T(0); }
 
   // Software pipeline: the operands of step k0 + kBK are fetched into
   // registers while the MFMAs of step k0 run from LDS, so the global-load
   // latency overlaps the matrix work instead of preceding it.
-  constexpr int kPer = kBK * TILE / kGemmThreads;  // elements per thread and operand
-  cplx<T> ra[kPer], rb[kPer];
-  // element e of the kBK x TILE operand tiles of step K0
+  constexpr int kPerA = kBK * TM / kGemmThreads;  // elements per thread
+  constexpr int kPerB = kBK * TN / kGemmThreads;
+  static_assert(kPerA >= 1 && kPerB >= 1, "tile too small for the workgroup");
+  cplx<T> ra[kPerA], rb[kPerB];
+  // element e of the kBK x TM / kBK x TN operand tiles of step K0
   auto load_a = [=](int e, int64_t k0) -> cplx<T> {
-    const int kk = CONJ_A ? (e & (kBK - 1)) : (e / TILE);
-    const int mm = CONJ_A ? (e / kBK) : (e & (TILE - 1));
+    const int kk = CONJ_A ? (e & (kBK - 1)) : (e / TM);
+    const int mm = CONJ_A ? (e / kBK) : (e % TM);
     const int64_t ka = k0 + kk, m = m0 + mm;
     if (ka >= kend || m >= M) return zero;
     return CONJ_A ? A[ka + m * lda] : A[m + ka * lda];
   };
   auto load_b = [=](int e, int64_t k0) -> cplx<T> {
-    const int kb = CONJ_B ? (e / TILE) : (e & (kBK - 1));
-    const int nb = CONJ_B ? (e & (TILE - 1)) : (e / kBK);
+    const int kb = CONJ_B ? (e / TN) : (e & (kBK - 1));
+    const int nb = CONJ_B ? (e % TN) : (e / kBK);
     const int64_t kq = k0 + kb, n = n0 + nb;
     if (kq >= kend || n >= N) return zero;
     if (CONJ_B) {  // op(B) = B^H, B stored n x k
@@ -105,42 +117,44 @@ __global__ __launch_bounds__(kGemmThreads) void k_cgemm(
 
   if (kbeg < kend) {
 #pragma unroll
-    for (int p = 0; p < kPer; ++p) {
-      ra[p] = load_a(tid + p * kGemmThreads, kbeg);
-      rb[p] = load_b(tid + p * kGemmThreads, kbeg);
-    }
+    for (int p = 0; p < kPerA; ++p) ra[p] = load_a(tid + p * kGemmThreads, kbeg);
+#pragma unroll
+    for (int p = 0; p < kPerB; ++p) rb[p] = load_b(tid + p * kGemmThreads, kbeg);
   }
   for (int64_t k0 = kbeg; k0 < kend; k0 += kBK) {
 #pragma unroll
-    for (int p = 0; p < kPer; ++p) {
+    for (int p = 0; p < kPerA; ++p) {
       const int e = tid + p * kGemmThreads;
       if (CONJ_A) As[e & (kBK - 1)][e / kBK] = ra[p];
-      else As[e / TILE][e & (TILE - 1)] = ra[p];
-      if (CONJ_B) Bs[e / TILE][e & (TILE - 1)] = rb[p];
+      else As[e / TM][e % TM] = ra[p];
+    }
+#pragma unroll
+    for (int p = 0; p < kPerB; ++p) {
+      const int e = tid + p * kGemmThreads;
+      if (CONJ_B) Bs[e / TN][e % TN] = rb[p];
       else Bs[e & (kBK - 1)][e / kBK] = rb[p];
     }
     __syncthreads();
     if (k0 + kBK < kend) {
 #pragma unroll
-      for (int p = 0; p < kPer; ++p) {
-        ra[p] = load_a(tid + p * kGemmThreads, k0 + kBK);
-        rb[p] = load_b(tid + p * kGemmThreads, k0 + kBK);
-      }
+      for (int p = 0; p < kPerA; ++p) ra[p] = load_a(tid + p * kGemmThreads, k0 + kBK);
+#pragma unroll
+      for (int p = 0; p < kPerB; ++p) rb[p] = load_b(tid + p * kGemmThreads, k0 + kBK);
     }
 #pragma unroll
     for (int ks = 0; ks < kBK; ks += 4) {
       const int kk = ks + (lane >> 4);
-      cplx<T> a[NB], b[NB];
+      cplx<T> a[NBM], b[NBN];
 #pragma unroll
-      for (int i = 0; i < NB; ++i) a[i] = As[kk][wm + 16 * i + (lane & 15)];
+      for (int i = 0; i < NBM; ++i) a[i] = As[kk][wm + 16 * i + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < NB; ++j) b[j] = Bs[kk][wn + 16 * j + (lane & 15)];
+      for (int j = 0; j < NBN; ++j) b[j] = Bs[kk][wn + 16 * j + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < NB; ++i) {
+      for (int i = 0; i < NBM; ++i) {
         const T ai_re = CONJ_A ? a[i].y : -a[i].y;
         const T ai_im = CONJ_A ? -a[i].y : a[i].y;
 #pragma unroll
-        for (int j = 0; j < NB; ++j) {
+        for (int j = 0; j < NBN; ++j) {
           re[i][j] = Mfma<T>::op(a[i].x, b[j].x, re[i][j]);
           re[i][j] = Mfma<T>::op(ai_re, b[j].y, re[i][j]);
           im[i][j] = Mfma<T>::op(a[i].x, b[j].y, im[i][j]);
@@ -153,9 +167,9 @@ __global__ __launch_bounds__(kGemmThreads) void k_cgemm(
 
   const bool direct = gridDim.z == 1;
 #pragma unroll
-  for (int i = 0; i < NB; ++i)
+  for (int i = 0; i < NBM; ++i)
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
+    for (int j = 0; j < NBN; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t row = m0 + wm + 16 * i + Mfma<T>::row(lane, v);
@@ -307,18 +321,76 @@ static GemvPlan gemv_plan(bool ca, int64_t m, int64_t k) {
   return {(k + kchunk - 1) / kchunk, kchunk};
 }
 
+// Tile variants of k_cgemm: (TM, TN, WAVES_M).
+struct GemmVariant {
+  int tm, tn, waves_m;
+};
+constexpr GemmVariant kVariants[] = {
+    {32, 32, 2},    // 0: small outputs (the reduced problem), wave 16x16
+    {64, 64, 2},    // 1: square-ish outputs, wave 32x32
+    {32, 128, 1},   // 2: tall outputs with up to 128 columns: A streamed once
+    {64, 128, 2},   // 3: tall outputs, wave 32x64
+    {128, 64, 2},   // 4: tall-row outputs (A^H B with m > n), wave 64x32
+    {128, 32, 4},   // 5: tall outputs with 32 columns
+    {32, 64, 1},    // 6: tall outputs with up to 64 columns, wave 32x16
+};
+constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+
 struct SplitPlan {
   int64_t splits;
   int64_t kchunk;
-  int tile;
+  int variant;
 };
 
-static SplitPlan split_plan(int op_a, int64_t m, int64_t n, int64_t k) {
-  const int64_t tiles64 = ((m + 63) / 64) * ((n + 63) / 64);
-  // outputs of fewer than 256 64x64 tiles take 32x32 tiles, except the tall
-  // A^H B products over npw, whose parallelism comes from splitting K
-  const int tile = (tiles64 < 256 && !(op_a == QEPW_OP_C && k > 2048)) ? 32 : 64;
-  const int64_t tiles = ((m + tile - 1) / tile) * ((n + tile - 1) / tile);
+// Tuned choices (qepw_gemm_set_rule): the first rule whose ranges hold the
+// shape wins over the built-in heuristic.
+struct GemmRule {
+  int precision, op;
+  int64_t m_lo, m_hi, n_lo, n_hi, k_lo, k_hi;
+  int variant;
+  int64_t splits;
+};
+constexpr int kMaxRules = 256;
+GemmRule g_rules[kMaxRules];
+int g_nrules = 0;
+
+static int64_t tiles_of(int variant, int64_t m, int64_t n) {
+  const GemmVariant& v = kVariants[variant];
+  return ((m + v.tm - 1) / v.tm) * ((n + v.tn - 1) / v.tn);
+}
+
+static SplitPlan finish_plan(int variant, int64_t splits, int64_t k) {
+  if (splits < 1) splits = 1;
+  int64_t kchunk = (k + splits - 1) / splits;
+  kchunk = ((kchunk + kBK - 1) / kBK) * kBK;
+  if (kchunk < kBK) kchunk = kBK;
+  splits = (k + kchunk - 1) / kchunk;
+  if (splits < 1) splits = 1;
+  return {splits, kchunk, variant};
+}
+
+static SplitPlan split_plan(int precision, int op_a, int64_t m, int64_t n, int64_t k) {
+  for (int r = 0; r < g_nrules; ++r) {
+    const GemmRule& g = g_rules[r];
+    if (g.precision == precision && g.op == op_a && m >= g.m_lo && m <= g.m_hi &&
+        n >= g.n_lo && n <= g.n_hi && k >= g.k_lo && k <= g.k_hi)
+      return finish_plan(g.variant, g.splits, k);
+  }
+  const bool ca = (op_a & QEPW_OP_C) != 0;
+  const bool tall = ca && k > 2048;
+  int variant;
+  if (tall) {
+    // A^H B over npw: parallelism from splitting K
+    variant = 1;
+  } else if (!ca && m > 4096 && n <= 128) {
+    // npw x n outputs (Ritz vectors, residuals, projector sums): all n
+    // columns in one workgroup, so the npw x k operand is read once
+    variant = n > 64 ? 2 : (n > 32 ? 6 : 0);
+  } else {
+    const int64_t tiles64 = ((m + 63) / 64) * ((n + 63) / 64);
+    variant = tiles64 < 256 ? 0 : 1;
+  }
+  const int64_t tiles = tiles_of(variant, m, n);
   int64_t splits = 1;
   // split K when the output tiles alone cannot fill the 256 CUs: the tall
   // A^H B products over npw (about two workgroups per CU and at least 256 of
@@ -326,19 +398,36 @@ static SplitPlan split_plan(int op_a, int64_t m, int64_t n, int64_t k) {
   // partial tiles than on the product) and the small square products of the
   // reduced eigensolve (small partial tiles: four workgroups per CU, at
   // least 128 of K per split)
-  const bool tall = op_a == QEPW_OP_C && k > 2048;
-  if (tall || (op_a == QEPW_OP_C && k > 512) || (tiles < 256 && k >= 256)) {
+  if (tall || (ca && k > 512) || (tiles < 256 && k >= 256)) {
     const int64_t want = ((tall ? 512 : 1024) + tiles - 1) / tiles;
     const int64_t most = k / (tall ? 256 : 128);
     splits = want < most ? want : most;
-    if (splits < 1) splits = 1;
   }
-  int64_t kchunk = (k + splits - 1) / splits;
-  kchunk = ((kchunk + kBK - 1) / kBK) * kBK;
-  if (kchunk < kBK) kchunk = kBK;
-  splits = (k + kchunk - 1) / kchunk;
-  if (splits < 1) splits = 1;
-  return {splits, kchunk, tile};
+  return finish_plan(variant, splits, k);
+}
+
+int gemm_set_rule(int precision, int op_a, const int64_t* ranges, int variant, int64_t splits) {
+  if (precision != QEPW_C128 && precision != QEPW_C64) return QEPW_ERR_ARG;
+  if (op_a & ~(QEPW_OP_C | QEPW_OP_BC)) return QEPW_ERR_ARG;
+  if (variant < 0 || variant >= kNumVariants || splits < 1 || splits > 65535 || !ranges)
+    return QEPW_ERR_ARG;
+  if (g_nrules >= kMaxRules) return QEPW_ERR_UNSUPPORTED;
+  g_rules[g_nrules++] = {precision, op_a,      ranges[0], ranges[1], ranges[2],
+                         ranges[3], ranges[4], ranges[5], variant,   splits};
+  return QEPW_OK;
+}
+
+void gemm_clear_rules() { g_nrules = 0; }
+
+int gemm_variant_count() { return kNumVariants; }
+
+int gemm_plan(int precision, int op_a, int64_t m, int64_t n, int64_t k, int64_t* out) {
+  const SplitPlan sp = split_plan(precision, op_a & (QEPW_OP_C | QEPW_OP_BC), m, n, k);
+  out[0] = sp.variant;
+  out[1] = sp.splits;
+  out[2] = kVariants[sp.variant].tm;
+  out[3] = kVariants[sp.variant].tn;
+  return QEPW_OK;
 }
 
 // Products of commuting Hermitian matrices (the Newton-Schulz steps of the
@@ -467,7 +556,7 @@ size_t gemm_workspace_bytes(int precision, int op, int64_t m, int64_t n, int64_t
   const size_t elem = precision == QEPW_C128 ? sizeof(double2) : sizeof(float2);
   if (n == 1 && k > 0 && m > 0)
     return (size_t)gemv_plan((op & QEPW_OP_C) != 0, m, k).splits * (size_t)m * elem;
-  const SplitPlan sp = split_plan(op & QEPW_OP_C, m, n, k);
+  const SplitPlan sp = split_plan(precision, op & (QEPW_OP_C | QEPW_OP_BC), m, n, k);
   if (sp.splits <= 1) return 0;
   const size_t c = precision == QEPW_C128 ? sizeof(double2) : sizeof(float2);
   return (size_t)sp.splits * (size_t)m * (size_t)n * c;
@@ -514,24 +603,35 @@ static int gemm_t(int op, int64_t m, int64_t n, int64_t k, const double* alpha,
                   void* c, int64_t ldc, void* ws, size_t ws_bytes, hipStream_t stream) {
   if (n == 1 && k > 0)
     return gemv_t<T>(op, m, k, alpha, a, lda, b, ldb, beta, c, ldc, ws, ws_bytes, stream);
-  const SplitPlan sp = split_plan(op & QEPW_OP_C, m, n, k);
+  const SplitPlan sp = split_plan(std::is_same<T, double>::value ? QEPW_C128 : QEPW_C64,
+                                  op & (QEPW_OP_C | QEPW_OP_BC), m, n, k);
   if (sp.splits > 65535) return QEPW_ERR_UNSUPPORTED;
   const size_t need = sp.splits > 1 ? (size_t)sp.splits * m * n * sizeof(cplx<T>) : 0;
   if (ws_bytes < need) return QEPW_ERR_WORKSPACE;
-  const dim3 grid((unsigned)((m + sp.tile - 1) / sp.tile),
-                  (unsigned)((n + sp.tile - 1) / sp.tile), (unsigned)sp.splits);
+  const GemmVariant gv = kVariants[sp.variant];
+  if ((m + gv.tm - 1) / gv.tm > 0x7fffffff || (n + gv.tn - 1) / gv.tn > 65535)
+    return QEPW_ERR_UNSUPPORTED;
+  const dim3 grid((unsigned)((m + gv.tm - 1) / gv.tm), (unsigned)((n + gv.tn - 1) / gv.tn),
+                  (unsigned)sp.splits);
   const T ar = (T)alpha[0], ai = (T)alpha[1], br = (T)beta[0], bi = (T)beta[1];
   cplx<T>* part = static_cast<cplx<T>*>(ws);
   const cplx<T>* pa = static_cast<const cplx<T>*>(a);
   const cplx<T>* pb = static_cast<const cplx<T>*>(b);
   cplx<T>* pc = static_cast<cplx<T>*>(c);
-#define QEPW_GEMM_TILE(CA, CB, TL)                                                         \
-  hipLaunchKernelGGL((k_cgemm<T, CA, CB, TL>), grid, dim3(kGemmThreads), 0, stream, m, n, \
-                     k, sp.kchunk, pa, lda, pb, ldb, pc, ldc, part, ar, ai, br, bi)
-#define QEPW_GEMM_LAUNCH(CA, CB)                   \
-  do {                                             \
-    if (sp.tile == 32) QEPW_GEMM_TILE(CA, CB, 32); \
-    else QEPW_GEMM_TILE(CA, CB, 64);               \
+#define QEPW_GEMM_TILE(CA, CB, TM, TN, WM)                                                \
+  hipLaunchKernelGGL((k_cgemm<T, CA, CB, TM, TN, WM>), grid, dim3(kGemmThreads), 0, stream, \
+                     m, n, k, sp.kchunk, pa, lda, pb, ldb, pc, ldc, part, ar, ai, br, bi)
+#define QEPW_GEMM_LAUNCH(CA, CB)                                   \
+  do {                                                             \
+    switch (sp.variant) {                                          \
+      case 0: QEPW_GEMM_TILE(CA, CB, 32, 32, 2); break;            \
+      case 1: QEPW_GEMM_TILE(CA, CB, 64, 64, 2); break;            \
+      case 2: QEPW_GEMM_TILE(CA, CB, 32, 128, 1); break;           \
+      case 3: QEPW_GEMM_TILE(CA, CB, 64, 128, 2); break;           \
+      case 4: QEPW_GEMM_TILE(CA, CB, 128, 64, 2); break;           \
+      case 5: QEPW_GEMM_TILE(CA, CB, 128, 32, 4); break;           \
+      default: QEPW_GEMM_TILE(CA, CB, 32, 64, 1); break;           \
+    }                                                              \
   } while (0)
   const bool ca = (op & QEPW_OP_C) != 0, cb = (op & QEPW_OP_BC) != 0;
   if (ca && cb) QEPW_GEMM_LAUNCH(true, true);
@@ -558,8 +658,6 @@ int gemm(int precision, int op, int64_t m, int64_t n, int64_t k, const double* a
   const bool ca = (op & QEPW_OP_C) != 0, cb = (op & QEPW_OP_BC) != 0;
   if (ldc < m || ldb < (cb ? n : (k > 0 ? k : 1))) return QEPW_ERR_ARG;
   if (!ca ? lda < m : lda < (k > 0 ? k : 1)) return QEPW_ERR_ARG;
-  if ((m + kTile - 1) / kTile > 0x7fffffff || (n + kTile - 1) / kTile > 65535)
-    return QEPW_ERR_UNSUPPORTED;
   if (precision == QEPW_C128)
     return gemm_t<double>(op, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ws, ws_bytes,
                           stream);

@@ -26,6 +26,10 @@ int inverse_real(const Plan& p, const void* grid_g, void* out_real, void* worksp
 
 // qepw_gemm.hip
 size_t gemm_workspace_bytes(int precision, int op_a, int64_t m, int64_t n, int64_t k);
+int gemm_set_rule(int precision, int op_a, const int64_t* ranges, int variant, int64_t splits);
+void gemm_clear_rules();
+int gemm_variant_count();
+int gemm_plan(int precision, int op_a, int64_t m, int64_t n, int64_t k, int64_t* out);
 int gemm(int precision, int op_a, int64_t m, int64_t n, int64_t k, const double* alpha,
          const void* a, int64_t lda, const void* b, int64_t ldb, const double* beta,
          void* c, int64_t ldc, void* workspace, size_t ws_bytes, hipStream_t stream);
@@ -53,6 +57,11 @@ int permute_rows(int src_precision, int dst_precision, int direction, const int*
                  hipStream_t stream);
 int row_real_dot(const void* a, int64_t lda, const void* b, int64_t ldb, int64_t nrows,
                  int ncols, double scale, double* acc, hipStream_t stream);
+int copy_columns(int src_precision, int dst_precision, const void* src, int64_t lds,
+                 const int* scol, void* dst, int64_t ldd, const int* dcol, const double* scale,
+                 int64_t nrows, int ncols, hipStream_t stream);
+int set_block(int precision, void* dst, int64_t ldd, int64_t nrows, int ncols, double diag,
+              const double* d, hipStream_t stream);
 int scale_rows(int precision, const void* src, int64_t lds, const double* s, int64_t sstride,
                const double* cfac, void* dst, int64_t ldd, int64_t nrows, int nb,
                hipStream_t stream);

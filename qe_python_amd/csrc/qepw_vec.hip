@@ -257,6 +257,35 @@ __global__ __launch_bounds__(kRedThreads) void k_projector_forces(
   if (threadIdx.x == 0) out[a * 3 + c] = 2.0 * s;
 }
 
+// dst[:, dcol(j)] = s_j src[:, scol(j)] for j < ncols (column maps and scale
+// optional; the precisions of src and dst may differ): the column gathers,
+// scatters, widenings and per-column scalings of the Davidson bookkeeping.
+template <typename TS, typename TD>
+__global__ void k_copy_columns(const cplx<TS>* __restrict__ src, int64_t lds,
+                               const int* __restrict__ scol, cplx<TD>* __restrict__ dst,
+                               int64_t ldd, const int* __restrict__ dcol,
+                               const double* __restrict__ scale, int64_t nrows) {
+  const int j = blockIdx.y;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrows) return;
+  const int64_t sc = scol ? scol[j] : j;
+  const int64_t dc = dcol ? dcol[j] : j;
+  const cplx<TS> v = src[sc * lds + g];
+  const double s = scale ? scale[j] : 1.0;
+  dst[dc * ldd + g] = mk<TD>((TD)((double)v.x * s), (TD)((double)v.y * s));
+}
+
+// dst[i, j] = (i == j) ? (d ? d[i] : diag) : 0 over an nrows x ncols block.
+template <typename T>
+__global__ void k_set_block(cplx<T>* __restrict__ dst, int64_t ldd, int64_t nrows, double diag,
+                            const double* __restrict__ d) {
+  const int64_t j = blockIdx.y;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  const double v = (i == j) ? (d ? d[i] : diag) : 0.0;
+  dst[j * ldd + i] = mk<T>((T)v, T(0));
+}
+
 static inline dim3 rows_grid(int64_t nrows, int nb) {
   return dim3((unsigned)((nrows + 255) / 256), (unsigned)nb);
 }
@@ -700,6 +729,37 @@ int projector_forces(int precision, const void* dO, int64_t lddo, int64_t dstrid
       hipLaunchKernelGGL(k_projector_forces<float>, dim3(natom, 3), dim3(kRedThreads), 0,
                          stream, static_cast<const float2*>(dO), lddo, dstride,
                          static_cast<const float2*>(DO), lddO, f, nb, off, out));
+  return status();
+}
+
+int copy_columns(int src_precision, int dst_precision, const void* src, int64_t lds,
+                 const int* scol, void* dst, int64_t ldd, const int* dcol, const double* scale,
+                 int64_t nrows, int ncols, hipStream_t stream) {
+  if (ncols <= 0 || nrows <= 0) return QEPW_OK;
+  if (ncols > 65535) return QEPW_ERR_UNSUPPORTED;
+  const dim3 g = rows_grid(nrows, ncols);
+#define QEPW_COPY_COLS(TS, TD)                                                              \
+  hipLaunchKernelGGL((k_copy_columns<TS, TD>), g, dim3(256), 0, stream,                    \
+                     static_cast<const cplx<TS>*>(src), lds, scol, static_cast<cplx<TD>*>(dst), \
+                     ldd, dcol, scale, nrows)
+  if (src_precision == QEPW_C128 && dst_precision == QEPW_C128) QEPW_COPY_COLS(double, double);
+  else if (src_precision == QEPW_C128 && dst_precision == QEPW_C64) QEPW_COPY_COLS(double, float);
+  else if (src_precision == QEPW_C64 && dst_precision == QEPW_C128) QEPW_COPY_COLS(float, double);
+  else if (src_precision == QEPW_C64 && dst_precision == QEPW_C64) QEPW_COPY_COLS(float, float);
+  else return QEPW_ERR_ARG;
+#undef QEPW_COPY_COLS
+  return status();
+}
+
+int set_block(int precision, void* dst, int64_t ldd, int64_t nrows, int ncols, double diag,
+              const double* d, hipStream_t stream) {
+  if (ncols <= 0 || nrows <= 0) return QEPW_OK;
+  if (ncols > 65535) return QEPW_ERR_UNSUPPORTED;
+  QEPW_DISPATCH(precision,
+      hipLaunchKernelGGL(k_set_block<double>, rows_grid(nrows, ncols), dim3(256), 0, stream,
+                         static_cast<double2*>(dst), ldd, nrows, diag, d),
+      hipLaunchKernelGGL(k_set_block<float>, rows_grid(nrows, ncols), dim3(256), 0, stream,
+                         static_cast<float2*>(dst), ldd, nrows, diag, d));
   return status();
 }
 

@@ -96,7 +96,7 @@ def _matrix(t: torch.Tensor):
         raise ValueError(_NOT_MATRIX)
     s0, s1 = shape
     st0, st1 = stride
-    if st1 != 1 and s1 > 1:
+    if st1 != 1 and s1 > 1 and s0 > 0:
         # the kernels read each column as s1 contiguous elements, one column
         # or many
         raise ValueError(_NOT_MATRIX)
@@ -130,6 +130,14 @@ class Workspace:
 
 # workspace bytes of a GEMM shape (a pure function of the shape)
 _GEMM_WORKSPACE: dict = {}
+
+
+def forget_gemm_plans() -> None:
+    """Drop the cached workspace sizes: a change of the library's tile rules
+    (qe_python_amd.autotune) changes the split plan of a shape."""
+    _GEMM_WORKSPACE.clear()
+
+
 # set to a list to record (class, dtype, m, n, k, start, stop) for every GEMM,
 # with HIP events on the launch stream around it; the class is the label of
 # the enclosing ``gemm_class`` scope of this thread ("other" outside one)
@@ -258,6 +266,37 @@ def permute_rows(direction: int, perm: torch.Tensor | None, src: torch.Tensor,
                                       src.data_ptr(), ld_of(src), dst.data_ptr(), ld_of(dst),
                                       nrows, src.shape[0], stream_handle(src.device))
     nat.check(rc, "qepw_permute_rows")
+
+
+def copy_columns(src: torch.Tensor, dst: torch.Tensor, nrows: int, ncols: int | None = None,
+                 src_cols: torch.Tensor | None = None, dst_cols: torch.Tensor | None = None,
+                 scale: torch.Tensor | None = None) -> None:
+    """dst[dst_cols[j]] = scale[j] * src[src_cols[j]] for the first ``ncols``
+    columns (column-major (ncols, ld) tensors; maps int32 and scale float64 on
+    the device, each optional), converting between complex128 and complex64."""
+    if ncols is None:
+        ncols = int(src_cols.shape[0]) if src_cols is not None else int(src.shape[0])
+    rc = nat.load().qepw_copy_columns(
+        precision_code(src.dtype), precision_code(dst.dtype), src.data_ptr(), ld_of(src),
+        None if src_cols is None else src_cols.data_ptr(), dst.data_ptr(), ld_of(dst),
+        None if dst_cols is None else dst_cols.data_ptr(),
+        None if scale is None else scale.data_ptr(), nrows, ncols, stream_handle(dst.device))
+    nat.check(rc, "qepw_copy_columns")
+
+
+def set_block(dst: torch.Tensor, nrows: int, ncols: int, diag: float = 0.0,
+              d: torch.Tensor | None = None) -> None:
+    """dst[:ncols, :nrows] (column-major) = zero with ``diag`` (or the device
+    float64 ``d``) on its diagonal."""
+    rc = nat.load().qepw_set_block(precision_code(dst.dtype), dst.data_ptr(), ld_of(dst), nrows,
+                                   ncols, float(diag), None if d is None else d.data_ptr(),
+                                   stream_handle(dst.device))
+    nat.check(rc, "qepw_set_block")
+
+
+def index_tensor(values, device: torch.device) -> torch.Tensor:
+    """Host integer indices -> device int32 (staged through pinned memory)."""
+    return upload(np.asarray(values, dtype=np.int32), device)
 
 
 def accumulate_projector_diagonal(coupled: torch.Tensor, beta: torch.Tensor, nrows: int,

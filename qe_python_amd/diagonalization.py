@@ -37,6 +37,7 @@ There is no CPU fallback: without a HIP device or without
 
 from __future__ import annotations
 
+import dataclasses
 from contextlib import nullcontext
 from dataclasses import dataclass
 from typing import Callable
@@ -70,9 +71,13 @@ MIXED_STALL_SPAN = 100.0
 # MIXED_LOCK_GAP) and continues on the others only
 MIXED_LOCK_MIN = 8
 MIXED_LOCK_GAP = 1.0e-3     # Ha
-# Gram-Schmidt passes of the complex64 phase's corrections against the basis
-# (one pass costs more iterations than the second pass saves)
-ORTHO_PASSES = 2
+# classical Gram-Schmidt passes of the complex64 phase's corrections against
+# the basis, before their Cholesky QR (on the medium workload a second pass
+# leaves the iteration count at 32 and costs ~4% of the solve)
+ORTHO_PASSES = 1
+# subspace multiplier of the complex64 phase (None: the caller's
+# subspace_multiplier, as the FP64 refinement always uses)
+MIXED_COARSE_MULTIPLIER = None
 # diagnostics: set to a list to receive one record per Davidson iteration
 # (storage dtype, iteration, unconverged roots, max |de|, max residual norm
 # or None when residuals are not formed)
@@ -518,9 +523,16 @@ def _upload_block(vectors, perm, npw, device, dtype):
 
 
 def _download_block(block, perm, npw):
+    """Reference-order complex128 host copy (npw, nb) of a device-order
+    block, copied into pinned memory (a pageable device-to-host copy runs at
+    a fraction of the link's rate); the array keeps its pinned buffer alive,
+    and torch's host allocator reuses it once the array is released."""
     ref = torch.empty((block.shape[0], npw), dtype=torch.complex128, device=block.device)
     dv.permute_rows(1, perm, block, ref, npw)
-    return ref.cpu().numpy().T
+    host = torch.empty(ref.shape, dtype=ref.dtype, pin_memory=True)
+    host.copy_(ref, non_blocking=True)
+    torch.cuda.current_stream(block.device).synchronize()
+    return host.numpy().T
 
 
 def _device_hamiltonian(operator, operator_into):
@@ -563,6 +575,10 @@ def _check_room(active: int, count: int, maximum_subspace: int) -> None:
                          f"initial vectors or a larger subspace_multiplier")
 
 
+def _host_dtype(dtype: torch.dtype):
+    return np.complex64 if dtype == torch.complex64 else np.complex128
+
+
 def _cholesky_orthonormalize(block: torch.Tensor, npw: int, ws, condition: float = 1.0e6):
     """Cholesky QR of a block of complex64 corrections: Gram = L L^H on the
     host (one thread), block L^-H on the device.  None when the Gram matrix
@@ -585,7 +601,7 @@ def _cholesky_orthonormalize(block: torch.Tensor, npw: int, ws, condition: float
         if d.min() <= 0.0 or (d.max() / d.min()) ** 2 > condition:
             return None
         inv = _lower_inverse(chol).conj().T   # L^-H
-    t_dev = dv.upload(inv.T, device).to(block.dtype)
+    t_dev = dv.upload(inv.T.astype(_host_dtype(block.dtype)), device)
     out = torch.empty_like(block)
     dv.gemm(nat.OP_N, npw, n, n, 1.0, block, t_dev, 0.0, out, ws)
     return out
@@ -604,7 +620,7 @@ def _orthonormalize_device(block: torch.Tensor, npw: int, ws, threshold: float =
         if float(sq.cpu()[0]) == 0.0:
             return block[:0].clone()
         out = torch.empty_like(block)
-        dv.normalize_selected_columns(block, sq, torch.zeros(1, dtype=torch.int32, device=device),
+        dv.normalize_selected_columns(block, sq, dv.index_tensor([0], device),
                                       out, npw)
         return out
     gram_t = torch.empty((n, n), dtype=block.dtype, device=device)
@@ -619,7 +635,7 @@ def _orthonormalize_device(block: torch.Tensor, npw: int, ws, threshold: float =
         return block[:0].clone()
     rank = int(np.count_nonzero(squared > threshold ** 2 * squared[0]))
     transform = rotation[:, order[:rank]] / np.sqrt(squared[:rank])[None, :]
-    t_dev = dv.upload(transform.T, device).to(block.dtype)
+    t_dev = dv.upload(transform.T.astype(_host_dtype(block.dtype)), device)
     out = torch.empty((rank, npw), dtype=block.dtype, device=device)
     dv.gemm(nat.OP_N, npw, rank, n, 1.0, block, t_dev, 0.0, out, ws)
     return out
@@ -757,8 +773,12 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
     # FP64 refinement what is left
     de_switch, res_switch = switch
     start = _upload_block(trial, perm_dev, npw, device, torch.complex64)
+    coarse_ctx = ctx
+    if MIXED_COARSE_MULTIPLIER is not None:
+        coarse_ctx = dataclasses.replace(
+            ctx, subspace_multiplier=min(ctx.subspace_multiplier, MIXED_COARSE_MULTIPLIER))
     coarse = _davidson_phase(
-        ctx, start, torch.complex64, np.maximum(energy_bounds, de_switch),
+        coarse_ctx, start, torch.complex64, np.maximum(energy_bounds, de_switch),
         np.full(number_of_roots, res_switch), None, np.maximum(root_tolerances, de_switch),
         max_iterations - 1, initial_is_ritz, initial_is_orthonormal, initial_eigenvalues,
         applied_block, perm_dev, stall_window=MIXED_STALL_WINDOW, require_residuals=True,
@@ -766,7 +786,8 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
     del start
     # ... then the FP64 refinement from its Ritz vectors; the reduced pencil
     # carries S, so the FP32-orthonormal block needs no re-orthonormalisation
-    start = coarse.X.to(torch.complex128)
+    start = torch.empty(coarse.X.shape, dtype=torch.complex128, device=device)
+    dv.copy_columns(coarse.X, start, npw)
     coarse.X = None
     fine = _davidson_phase(
         ctx, start, torch.complex128, energy_bounds, residual_bounds, residual_energy_scale,
@@ -851,7 +872,14 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
         allv = np.concatenate(lock_vals + [values])
         allr = np.concatenate(lock_res + [residual_norms])
         order = np.argsort(allv, kind="stable")
-        Xall = torch.cat([locked, X]).index_select(0, dv.upload(order.astype(np.int64), device))
+        # source column j (locked first, then X) lands at position pos[j]
+        pos = np.empty(len(order), dtype=np.int32)
+        pos[order] = np.arange(len(order), dtype=np.int32)
+        nl = int(locked.shape[0])
+        Xall = torch.empty((len(order), npw), dtype=dtype, device=device)
+        dv.copy_columns(locked, Xall, npw, nl, dst_cols=dv.index_tensor(pos[:nl], device))
+        dv.copy_columns(X, Xall, npw, len(order) - nl,
+                        dst_cols=dv.index_tensor(pos[nl:], device))
         return allv[order], Xall, allr[order]
     if start_is_ritz or start_is_orthonormal:
         basis = start[:number_of_roots]
@@ -889,7 +917,10 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
     def ritz_solve():
         with _measure(timers, "cdiaghg", device):
             roots, coef = rr.solve(active)
-            coef = coef.to(dtype)
+            if coef.dtype != dtype:
+                narrow = torch.empty(coef.shape, dtype=dtype, device=device)
+                dv.copy_columns(coef, narrow, int(coef.shape[1]))
+                coef = narrow
         return roots, coef
 
     def deflate(R):
@@ -919,10 +950,13 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
         """Residuals of the roots ``cols`` only, packed:
         R_u = HV C_u - V (C_u diag(e_u)) -- all the next expansion needs."""
         with _measure(timers, "cegterg:upda", device):
-            idx = dv.upload(cols.astype(np.int64), device)
-            cu = coef.index_select(0, idx)
+            idx = dv.index_tensor(cols, device)
             e = dv.upload(np.asarray(roots[cols], dtype=np.float64), device)
-            ce = cu * e.to(cu.real.dtype)[:, None]
+            # the unconverged roots' coefficients C_u and C_u diag(e_u)
+            cu = torch.empty((len(cols), active), dtype=dtype, device=device)
+            ce = torch.empty_like(cu)
+            dv.copy_columns(coef, cu, active, src_cols=idx)
+            dv.copy_columns(coef, ce, active, src_cols=idx, scale=e)
             R = torch.empty((len(cols), npw), dtype=dtype, device=device)
             dv.gemm(nat.OP_N, npw, len(cols), active, 1.0, HV[:active], cu, 0.0, R, ws)
             dv.gemm(nat.OP_N, npw, len(cols), active, -1.0, V[:active], ce, 1.0, R, ws)
@@ -943,7 +977,8 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
         dv.subtract_band_energies(R, X, torch.from_numpy(np.ascontiguousarray(values)).to(device),
                                   npw)
         # the start block is its own Ritz block: coefficients = identity
-        coef = torch.eye(number_of_roots, dtype=dtype, device=device)
+        coef = torch.empty((number_of_roots, number_of_roots), dtype=dtype, device=device)
+        dv.set_block(coef, number_of_roots, number_of_roots, 1.0)
     else:
         values, coef = ritz_solve()
         X, R = ritz_block(values, coef)
@@ -1087,15 +1122,18 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
                 old_theta = None
                 if coarse and len(done) >= MIXED_LOCK_MIN:
                     # lock the run; the search continues on the roots above it
+                    # the run and the rest are contiguous column ranges
                     rest = np.arange(run, number_of_roots)
-                    di = dv.upload(done.astype(np.int64), device)
-                    ri = dv.upload(rest.astype(np.int64), device)
-                    rows_done = X.index_select(0, di)
-                    locked = rows_done if locked is None else torch.cat([locked, rows_done])
+                    nl = 0 if locked is None else int(locked.shape[0])
+                    grown = torch.empty((nl + run, npw), dtype=dtype, device=device)
+                    if nl:
+                        grown[:nl].copy_(locked)
+                    grown[nl:].copy_(X[:run])
+                    locked = grown
                     lock_vals.append(values[done])
                     lock_res.append(residual_norms[done] if track
                                     else np.zeros(len(done)))
-                    X, R = X.index_select(0, ri), R.index_select(0, ri)
+                    X, R = X[run:], R[run:]
                     values, residual_norms = values[rest], residual_norms[rest]
                     gidx = gidx[rest]
                     eb = energy_bounds[gidx]
@@ -1122,7 +1160,9 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
                     # above the locked run
                     rr.seed(np.asarray(old_theta)[len(done):])
                 # the basis is the Ritz block itself
-                coef = torch.eye(number_of_roots, dtype=dtype, device=device)
+                coef = torch.empty((number_of_roots, number_of_roots), dtype=dtype,
+                                   device=device)
+                dv.set_block(coef, number_of_roots, number_of_roots, 1.0)
         elif X is None:
             R = selected_residuals(values, coef, np.flatnonzero(unconverged))
             packed = True
@@ -1141,8 +1181,6 @@ def _result_vectors(X, perm_dev, npw, device_output=False):
         ref = torch.empty((X.shape[0], npw), dtype=torch.complex128, device=X.device)
         dv.permute_rows(1, perm_dev, X, ref, npw)
         return ref.T
-    if perm_dev is None:
-        return np.asfortranarray(X.to(torch.complex128).cpu().numpy().T)
     return np.asfortranarray(_download_block(X, perm_dev, npw))
 
 

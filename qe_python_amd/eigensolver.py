@@ -23,7 +23,7 @@ iteration with LAPACK zhegvd on the host (``generalized_eigh``,
 4. Fallback: at the first Rayleigh-Ritz, when fewer than nroots + 2 Ritz
    values of the previous solve are known, or when the sign iteration has
    not converged (an eigenvalue sits at sigma), the standardised A is
-   diagonalised densely on the device (``torch.linalg.eigh``, FP64).
+   copied to the host and diagonalised densely (LAPACK heevd, FP64).
    ``spectral=False`` instead copies the pencil to the host and solves it by
    zhegvd exactly as the reference does.
 
@@ -50,8 +50,6 @@ from . import _native as nat
 from . import device as dv
 
 C128 = torch.complex128
-# dense fallback on the host up to this size (see DeviceRayleighRitz._dense)
-HOST_DENSE_MAX = 224
 _THREADPOOLS = None
 
 
@@ -84,6 +82,24 @@ def _scaled_ns_alphas(lower: float, tol: float = 1e-15, limit: int = 60) -> list
     return alphas
 
 
+_GAUSS: dict = {}
+
+
+def _gaussian_columns(size: int, device) -> torch.Tensor:
+    """A fixed seeded complex Gaussian (size, size) block on the device,
+    drawn and uploaded once per process: the range sample only has to be
+    generic, and reusing it keeps the host out of the loop and the solves
+    reproducible."""
+    key = (str(device), int(size))
+    block = _GAUSS.get(key)
+    if block is None:
+        rng = np.random.default_rng(1234)
+        g = (rng.standard_normal((size, size)) + 1j * rng.standard_normal((size, size)))
+        block = dv.upload(g / np.sqrt(2.0), device)
+        _GAUSS[key] = block
+    return block
+
+
 class DeviceRayleighRitz:
     """The device-resident reduced problem of one Davidson solve."""
 
@@ -98,13 +114,22 @@ class DeviceRayleighRitz:
         self.spectral = spectral
         self.extra = int(extra)
         self.check = check
-        z = lambda: torch.zeros((self.maxsub, self.maxsub), dtype=C128, device=device)  # noqa: E731
+        def z():
+            m = torch.empty((self.maxsub, self.maxsub), dtype=C128, device=device)
+            dv.set_block(m, self.maxsub, self.maxsub)
+            return m
         # column-major matrices: tensor[c, r] holds element (r, c)
         self.H, self.S, self.W, self.A = z(), z(), z(), z()
         self.X, self.X2, self.Y = z(), z(), z()
-        # bound[0] and the column sums behind it; sig[0:2] and its partial sums
-        self.bound = torch.zeros(self.maxsub + 1, dtype=torch.float64, device=device)
-        self.sig = torch.zeros(self.maxsub + 2, dtype=torch.float64, device=device)
+        # read-back buffer of the spectral solve: sig[0:2] and its partial
+        # sums, bound[0] and the column sums behind it (written by
+        # qepw_sign_check / qepw_shift_scale before any read), then the Gram
+        # matrix of the range sample -- one contiguous prefix per read-back
+        self._head = 2 * self.maxsub + 4
+        self.rb = torch.empty(self._head + 2 * self.maxsub * self.maxsub, dtype=torch.float64,
+                              device=device)
+        self.sig = self.rb[:self.maxsub + 2]
+        self.bound = self.rb[self.maxsub + 2:2 * self.maxsub + 3]
         self.standard = 0            # W and A are valid for [:standard]
         self.theta = None            # previous Ritz values (host, ascending)
         self.sigma = None            # shift of the previous spectral solve
@@ -116,7 +141,6 @@ class DeviceRayleighRitz:
         self.stats = {"spectral": 0, "dense": 0, "host": 0, "ns_iterations": 0,
                       "max_check_error": 0.0, "skipped": 0, "ns_unconverged": 0,
                       "too_few_below": 0, "gram_not_pd": 0}
-        self._gauss = None           # Gaussian sample columns (device, drawn once)
 
     # -- views -------------------------------------------------------------
     def block(self, mat: torch.Tensor, r0: int, r1: int, c0: int, c1: int) -> torch.Tensor:
@@ -144,7 +168,7 @@ class DeviceRayleighRitz:
                 rows = torch.empty((active, nnew), dtype=V.dtype, device=self.device)
                 self._gemm(nat.OP_C, nnew, active, npw, 1.0, V[first:active], right[:active],
                            0.0, rows)
-                target.copy_(rows)
+                dv.copy_columns(rows, target, nnew, active)
             rc = nat.load().qepw_hermitian_complete(mat.data_ptr(), self.maxsub, first, active,
                                                     dv.stream_handle(self.device))
             nat.check(rc, "qepw_hermitian_complete")
@@ -167,10 +191,8 @@ class DeviceRayleighRitz:
         self._last_k = np.asarray(values[:self.nroots], dtype=np.float64).copy()
         # the Ritz vectors are the new basis: the eigenvectors of A = diag
         if self.u_prev is None:
-            self.u_prev = torch.zeros((self.nroots, self.maxsub), dtype=C128, device=self.device)
-        self.u_prev.zero_()
-        idx = torch.arange(min(k, self.u_prev.shape[0]), device=self.device)
-        self.u_prev[idx, idx] = 1.0
+            self.u_prev = torch.empty((self.nroots, self.maxsub), dtype=C128, device=self.device)
+        dv.set_block(self.u_prev, self.maxsub, self.nroots, 1.0)
 
     def seed(self, values: np.ndarray):
         """Ritz-value estimates (ascending) of the pencil to come, e.g. those
@@ -193,7 +215,8 @@ class DeviceRayleighRitz:
             Z = torch.empty((nnew, first), dtype=C128, device=self.device)
             self._gemm(nat.OP_N, first, nnew, first, 1.0, self.block(self.W, 0, first, 0, first),
                        self.block(self.S, 0, first, first, active), 0.0, Z)
-            M = self.block(self.S, first, active, first, active).clone()
+            M = torch.empty((nnew, nnew), dtype=C128, device=self.device)
+            dv.copy_columns(self.block(self.S, first, active, first, active), M, nnew, nnew)
             self._gemm(nat.OP_C, nnew, nnew, first, -1.0, Z, Z, 1.0, M)
             m = M.cpu().numpy().T
         m = 0.5 * (m + m.conj().T)
@@ -203,9 +226,10 @@ class DeviceRayleighRitz:
             raise np.linalg.LinAlgError(f"zhegvd failed: overlap not positive definite ({exc})")
         w22 = _lower_inverse(l22)
         w22_dev = dv.upload(w22.T, self.device)
-        self.block(self.W, first, active, first, active).copy_(w22_dev)
+        dv.copy_columns(w22_dev, self.block(self.W, first, active, first, active), nnew, nnew)
         # the strictly upper part of the new columns stays zero
-        self.block(self.W, 0, first, first, active).zero_()
+        if first > 0:
+            dv.set_block(self.block(self.W, 0, first, first, active), first, nnew)
         if first > 0:
             # W21 = -W22 (Z^H W11)
             T = torch.empty((first, nnew), dtype=C128, device=self.device)
@@ -269,27 +293,17 @@ class DeviceRayleighRitz:
         return w[:k].copy(), coef, w.copy()
 
     def _dense(self, n):
-        """All eigenpairs of the standardised A; C = W^H U.  Up to
-        ``HOST_DENSE_MAX`` rows the matrix is diagonalised on the host (one
-        BLAS thread, ~2 ms at 150), above on the device (rocSOLVER through
-        ``torch.linalg.eigh``, ~8-15 ms at 256-512, flat in n)."""
+        """All eigenpairs of the standardised A on the host (LAPACK heevd on
+        one BLAS thread, ~2 ms at 150 rows; the first solve of a pencil and
+        the fallbacks); C = W^H U on the device."""
         k = self.nroots
-        if n <= HOST_DENSE_MAX:
-            a = self.host_block(self.A, n)
-            w, v = self._host_eigh(0.5 * (a + a.conj().T))
-            U = dv.upload(v[:, :k].T, self.device)
-            w = torch.from_numpy(w)
-        else:
-            # the column-major block is stored transposed: tensor = A^T =
-            # conj(A) (A Hermitian), whose eigenvectors are the conjugates of
-            # A's; conj_physical, because a lazy conj() view would hand the
-            # kernels the unconjugated memory.  U: (k, n), column-major n x k
-            w, vt = torch.linalg.eigh(self.A[:n, :n])
-            U = torch.conj_physical(vt[:, :k].T.contiguous())
+        a = self.host_block(self.A, n)
+        w, v = self._host_eigh(0.5 * (a + a.conj().T))
+        U = dv.upload(np.ascontiguousarray(v[:, :k].T), self.device)
         coef = torch.empty((k, n), dtype=C128, device=self.device)
         self._gemm(nat.OP_C, n, k, n, 1.0, self.block(self.W, 0, n, 0, n), U, 0.0, coef)
         self._keep_vectors(U, n)
-        values = w.cpu().numpy().astype(np.float64)
+        values = np.asarray(w, dtype=np.float64)
         self._note_drift(values)
         self.theta = values
         self.sigma = None
@@ -323,9 +337,9 @@ class DeviceRayleighRitz:
     def _keep_vectors(self, U, n):
         k = self.nroots
         if self.u_prev is None:
-            self.u_prev = torch.zeros((k, self.maxsub), dtype=C128, device=self.device)
-        self.u_prev.zero_()
-        self.u_prev[:, :n].copy_(U)
+            self.u_prev = torch.empty((k, self.maxsub), dtype=C128, device=self.device)
+        dv.set_block(self.u_prev, self.maxsub, k)
+        dv.copy_columns(U, self.u_prev, n, k)
 
     def _spectral(self, n):
         k = self.nroots
@@ -413,40 +427,39 @@ class DeviceRayleighRitz:
 
         def sampled_range(count):
             """Q0 = (I - X) Omega / 2 for Omega = [previous eigenvectors,
-            Gaussian columns] (count columns) and its Gram matrix."""
+            Gaussian columns] (count columns) and its Gram matrix, with the
+            sign check and the exact bound ahead of it in one device buffer
+            (a single read-back)."""
             # rows [:count] and columns [:n] are all written below
             omega = torch.empty((count, self.maxsub), dtype=C128, device=self.device)
             start = 0
             if self.u_prev is not None:
                 start = min(k, count)
-                omega[:start].copy_(self.u_prev[:start])
+                dv.copy_columns(self.u_prev, omega, n, start)
             if start < count:
-                if self._gauss is None:
-                    # one seeded draw per reduced problem, on the device: the
-                    # sample only has to be generic, and reusing it keeps the
-                    # host out of the loop (and runs reproducible)
-                    gen = torch.Generator(device=self.device)
-                    gen.manual_seed(1234)
-                    self._gauss = torch.randn((self.maxsub, self.maxsub), dtype=C128,
-                                              device=self.device, generator=gen)
-                omega[start:count, :n].copy_(self._gauss[start:count, :n])
+                gauss = _gaussian_columns(self.maxsub, self.device)
+                dv.copy_columns(gauss[start:count], omega[start:count], n, count - start)
             Om = omega[:, :n]
-            Q0 = Om.clone()
+            Q0 = torch.empty((count, n), dtype=C128, device=self.device)
+            dv.copy_columns(Om, Q0, n, count)
             self._gemm(nat.OP_N, n, count, n, -0.5, X, Om, 0.5, Q0)
-            G = torch.empty((count, count), dtype=C128, device=self.device)
+            h = self._head
+            G = torch.view_as_complex(self.rb[h:h + 2 * count * count].view(count, count, 2))
             self._gemm(nat.OP_C, count, count, n, 1.0, Q0, Q0, 0.0, G)
-            return Q0, G
+            return Q0, self.rb[:h + 2 * count * count]
+
+        def unpack(rb, count):
+            host = rb.cpu().numpy()
+            g = host[self._head:].view(np.complex128).reshape(count, count).T
+            return (float(host[0]), float(host[1]), float(host[self.maxsub + 2]), g)
 
         # optimistic path: one read-back carries the sign check, the exact
         # bound and the Gram matrix of a range sample sized for the expected
         # number of eigenvalues below sigma
         sign_steps(alphas)
         count = min(n, len(theta) + 24)
-        Q0, G = sampled_range(count)
-        packed = torch.cat([self.sig[:2], self.bound[:1],
-                            torch.view_as_real(G).reshape(-1)]).cpu().numpy()
-        res2, trace, self._bound_prev = (float(v) for v in packed[:3])
-        g = packed[3:].view(np.complex128).reshape(count, count).T
+        Q0, rb = sampled_range(count)
+        res2, trace, self._bound_prev, g = unpack(rb, count)
         attempt = 0
         self._slow_shift = res2 >= 1e-20 * n
         while res2 >= 1e-20 * n and attempt < 2:
@@ -462,9 +475,9 @@ class DeviceRayleighRitz:
             self.stats["too_few_below"] += 1
             return None
         if g is None or m_below > count:
-            count = m_below
-            Q0, G = sampled_range(count)
-            g = G.cpu().numpy().T
+            count = max(count, m_below)
+            Q0, rb = sampled_range(count)
+            _, _, _, g = unpack(rb, count)
         mp = m_below
         Q0 = Q0[:mp]
         g = g[:mp, :mp]

@@ -85,3 +85,40 @@ def test_density_at_a_general_kpoint():
     expect = np.sum(w * np.real(np.einsum("gb,gb->b", coeff.conj(), local.apply(coeff))))
     got = np.sum(s.real_potential * rho) / rho.size
     assert abs(got - expect) <= 1e-11 * abs(expect)
+
+
+@pytest.mark.parametrize("shape", [(128, 128, 20), (144, 150, 16)])
+def test_density_on_split_stage_grids(shape):
+    """Planes no workgroup can hold in FP64 (128 x 128 and 144 x 150) take
+    the split plane stage in the density too: FP64 and FP32 against the
+    oracle, and the V-rho identity sum_r V rho / N = sum_b w_b <psi_b|V psi_b>
+    against the same grid's h_psi."""
+    import torch
+
+    from qe_python_amd.basis import LocalPotentialWorkspace, make_bases
+    from qe_python_amd.diagonalization import PlaneWaveHamiltonian
+    rng = np.random.default_rng(32)
+    reciprocal = 2.0 * np.pi / 12.0 * np.eye(3)
+    basis = make_bases(reciprocal, np.array([0.1, 0.3, 0.2]), 8.0)[0]
+    nb = 5
+    coeff = rng.standard_normal((len(basis), nb)) + 1j * rng.standard_normal((len(basis), nb))
+    w = rng.uniform(0.2, 2.0, nb)
+    ref = accumulate_density(coeff, w, linear_slots(basis.indices, shape), shape)
+    ws = LocalPotentialWorkspace(basis.indices, shape)
+    rho = np.zeros(shape)
+    ws.accumulate_density(rho, coeff, w)
+    np.testing.assert_allclose(rho, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    from qe_python_amd.diagonalization import _upload_block
+    dev = torch.device("cuda", torch.cuda.current_device())
+    psi32 = _upload_block(coeff, ws.plan.perm_device, ws.npw, dev, torch.complex64)
+    out32 = torch.empty(shape[::-1], dtype=torch.float64, device=dev)
+    ws._accumulate_device(psi32, torch.from_numpy(w).to(dev), out32)
+    np.testing.assert_allclose(out32.permute(2, 1, 0).cpu().numpy(), ref, rtol=0,
+                               atol=2e-6 * np.abs(ref).max())
+    vr = rng.uniform(-1.0, 0.5, shape)
+    ham = PlaneWaveHamiltonian(basis, np.zeros(shape, complex), (), real_potential=vr,
+                               local_workspace=ws)
+    hx = ham.apply(coeff) - coeff * np.asarray(basis.kinetic)[:, None]
+    lhs = float(np.sum(vr * rho)) / np.prod(shape)
+    rhs = float(np.sum(w * np.real(np.sum(coeff.conj() * hx, axis=0))))
+    assert lhs == pytest.approx(rhs, rel=1e-11)

@@ -67,6 +67,52 @@ def test_gemm_matches_numpy(prec, op, m, n, k):
     assert np.linalg.norm(got - ref) / scale < tol
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("op", ["N", "C", "NB", "CB"])
+def test_every_gemm_tile_variant_matches_numpy(prec, op):
+    """Each tile variant of the GEMM (forced through a tuning rule, with and
+    without split K) on shapes that are not multiples of its tiles, against
+    NumPy; the rules are cleared and the committed table restored after."""
+    import ctypes
+
+    from qe_python_amd import _native as nat
+    from qe_python_amd import autotune
+    from qe_python_amd import device as dv
+    lib = nat.load()
+    dtype = torch.complex128 if prec == "c128" else torch.complex64
+    code = (nat.OP_N if op[0] == "N" else nat.OP_C) | (nat.OP_BC if op.endswith("B") else 0)
+    ws = dv.Workspace(torch.device(DEV))
+    try:
+        for variant in range(lib.qepw_gemm_variant_count()):
+            for m, n, k, splits in [(70, 130, 45, 1), (4100, 97, 33, 1), (200, 61, 3000, 7),
+                                    (33, 129, 513, 3)]:
+                rng = np.random.default_rng(variant * 100 + m + n + k)
+                a = rand(rng, m, k) if op[0] == "N" else rand(rng, k, m)
+                b = rand(rng, n, k) if op.endswith("B") else rand(rng, k, n)
+                c = rand(rng, m, n)
+                alpha, beta = 0.6 + 0.3j, -0.2 + 0.1j
+                ta, tb, tc = dev_block(a, dtype), dev_block(b, dtype), dev_block(c, dtype)
+                lib.qepw_gemm_clear_rules()
+                rng_ = (ctypes.c_int64 * 6)(m, m, n, n, k, k)
+                nat.check(lib.qepw_gemm_set_rule(dv.precision_code(dtype), code, rng_,
+                                                 variant, splits), "set_rule")
+                dv.forget_gemm_plans()
+                plan = (ctypes.c_int64 * 4)()
+                nat.check(lib.qepw_gemm_plan(dv.precision_code(dtype), code, m, n, k, plan),
+                          "plan")
+                assert plan[0] == variant
+                dv.gemm(code, m, n, k, alpha, ta, tb, beta, tc, ws)
+                opa = a if op[0] == "N" else a.conj().T
+                bb = b.conj().T if op.endswith("B") else b
+                ref = alpha * (opa @ bb) + beta * c
+                tol = 1e-12 if prec == "c128" else 2e-6 + 1e-7 * np.sqrt(k)
+                err = np.linalg.norm(host_block(tc) - ref) / np.linalg.norm(ref)
+                assert err < tol, (variant, m, n, k, splits, err)
+    finally:
+        lib.qepw_gemm_clear_rules()
+        autotune.load_table()
+
+
 def test_gemm_split_k_is_deterministic():
     from qe_python_amd import _native as nat
     from qe_python_amd import device as dv
