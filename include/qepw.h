@@ -100,6 +100,25 @@ int qepw_apply_local(const qepw_plan* plan, int precision, const void* vectors,
                      void* stream);
 
 /*
+ * qepw_apply_local for ``count`` independent problems at once (k-points or
+ * cells with their own plans, bands, potentials, kinetics and workspaces; one
+ * precision): arrays of ``count`` entries, the arguments of qepw_apply_local
+ * per problem.  Problems whose grid lengths have a grouped kernel (square
+ * planes of side and z length 36, 48, 54, 60, 64, 72, 80 or 96, the team plane
+ * stage, all bands in one workspace) share three launches -- z-stage inverse,
+ * planes, z-stage forward -- for up to 24 problems; the others run one by
+ * one.  Results equal the one-problem calls bit for bit.  The batched form of
+ * the reference's per-k-point loop (qepy_pw/pw/scf.py:3340-3537) applying
+ * H on each k-point's bands (LocalPotentialWorkspace.apply,
+ * qepy_pw/basis.py:1354-1455).
+ */
+int qepw_apply_local_group(int count, const qepw_plan* const* plans, int precision,
+                           const void* const* vectors, const int64_t* ldv, const int* nb,
+                           const void* const* potentials, const void* const* kinetics,
+                           void* const* results, const int64_t* ldr, void* const* workspaces,
+                           const size_t* ws_bytes, void* stream);
+
+/*
  * out[z, x, y] = Re sum_G V(G) exp(+i G.r) (unnormalised backward FFT) of a
  * C-order (n1, n2, n3) complex128 grid, written in (n3, n2, n1) layout.
  * Replaces: inverse_real (qepy_pw/_native_fft.pyx:2188-2202) as used by

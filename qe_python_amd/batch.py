@@ -5,19 +5,28 @@ The reference solves the k-points of an SCF iteration one after another
 one MI355X a single 10k-24k plane-wave problem leaves the device idle while
 its host thread factorises the small FP64 reduced matrices, and its own
 launches are too small to fill 256 CUs late in a solve when few roots remain
-unconverged.  ``davidson_batch`` runs every problem's unchanged ``davidson``
-on its own HIP stream (default priority) from its own host thread:
+unconverged.  ``davidson_batch`` runs the problems together, in one of two
+schedules:
 
-* kernels of different problems overlap on the device (the streams map onto
-  the hardware queues), so uneven problems -- different FFT grids, npw,
-  root counts and iteration counts -- fill the GPU together;
-* the host-side FP64 work of one problem (Cholesky and eigensolve of the
-  sampled range, a few ms) runs on its own core with a single-threaded BLAS
-  while the device works for the others.
+* ``"rounds"`` (default): every problem's Davidson is the generator
+  ``davidson_steps``; ``schedule.run_many`` advances all of them in rounds
+  on one host thread and one stream.  In a round, H is applied to every
+  requested block of one precision at once -- the FFT phase of all problems
+  as one grouped launch per stage (``qepw_apply_local_group``: sticks,
+  planes, sticks, each workgroup picking its problem from a prefix table) --
+  and every requested host value (residual norms, reduced matrices) is read
+  back after one wait, so one problem's host factorisations run while the
+  device executes the others' kernels.
+* ``"threads"``: every problem's unchanged ``davidson`` on its own HIP
+  stream (default priority) from its own host thread; kernels of different
+  problems overlap through the hardware queues, each thread's host LAPACK
+  work runs on its own core with a single-threaded BLAS.
 
-Results are identical to solving the problems one by one: every kernel is
-deterministic (fixed-order reductions, no atomics) and each problem owns its
-Hamiltonian, workspace and reduced-problem state.
+Results are identical to solving the problems one by one in both schedules:
+every kernel is deterministic (fixed-order reductions, no atomics), the
+grouped FFT kernels run each problem's bands through the same code as the
+single-problem kernels, and each problem owns its Hamiltonian, workspace and
+reduced-problem state.
 """
 
 from __future__ import annotations
@@ -29,7 +38,8 @@ import torch
 
 from . import _native as nat
 from . import device as dv
-from .diagonalization import DavidsonResult, davidson
+from . import schedule as sched
+from .diagonalization import DavidsonResult, davidson, davidson_steps
 from .eigensolver import _one_blas_thread
 
 
@@ -61,19 +71,20 @@ class BatchProblem:
 
 
 def davidson_batch(problems, *, workers: int | None = None, device=None,
-                   **common) -> list[DavidsonResult]:
-    """Solve ``problems`` (a sequence of ``BatchProblem``) concurrently.
+                   schedule: str = "rounds", **common) -> list[DavidsonResult]:
+    """Solve ``problems`` (a sequence of ``BatchProblem``) together.
 
     ``common`` holds ``davidson`` keywords shared by all problems (a
-    problem's ``options`` override them).  ``workers`` host threads (default 2) each drive one problem at a time on their own
-    stream.  Device tensors returned with ``device_output=True`` are complete
-    when this returns.  Problems must not share an operator object (each
-    owns its device workspace).
-
-    The default of two workers is the measured optimum on one MI355X with
-    the 24-problem uneven workload (1.4x over solving one by one): beyond
-    two, the Python-level part of the Davidson loops contends for the GIL."""
+    problem's ``options`` override them).  ``schedule`` is "rounds" (all
+    problems in lock-step rounds with grouped H applications, on the
+    current stream) or "threads" (``workers`` host threads, default 2, each
+    driving one problem at a time on its own stream).  Device tensors
+    returned with ``device_output=True`` are complete when this returns.
+    Problems must not share an operator object (each owns its device
+    workspace)."""
     problems = list(problems)
+    if schedule not in ("rounds", "threads"):
+        raise ValueError("schedule must be 'rounds' or 'threads'")
     if not problems:
         return []
     owners = [getattr(p.operator_into or p.operator, "__self__", None) for p in problems]
@@ -82,6 +93,29 @@ def davidson_batch(problems, *, workers: int | None = None, device=None,
         raise ValueError("davidson_batch: problems must not share an operator")
     dev = dv.require_device(device)
     nat.load()
+    if schedule == "rounds":
+        return _batch_rounds(problems, dev, common)
+    return _batch_threads(problems, dev, workers, common)
+
+
+def _kwargs(problem, common) -> dict:
+    kwargs = dict(common)
+    kwargs.update(problem.options)
+    return kwargs
+
+
+def _batch_rounds(problems, dev, common) -> list[DavidsonResult]:
+    with torch.cuda.device(dev):
+        solves = [davidson_steps(p.operator, p.diagonal, p.number_of_roots,
+                                 initial_vectors=p.initial_vectors,
+                                 operator_into=p.operator_into, **_kwargs(p, common))
+                  for p in problems]
+        results = sched.run_many(solves, dev)
+        torch.cuda.current_stream(dev).synchronize()
+    return results
+
+
+def _batch_threads(problems, dev, workers, common) -> list[DavidsonResult]:
     workers = max(1, min(len(problems), 2 if workers is None else int(workers)))
     streams = _worker_streams(dev, workers)
     # operands uploaded on the caller's stream must be complete before the
@@ -95,13 +129,12 @@ def davidson_batch(problems, *, workers: int | None = None, device=None,
         try:
             stream = streams[slot]
             problem = problems[index]
-            kwargs = dict(common)
-            kwargs.update(problem.options)
             with torch.cuda.device(dev), torch.cuda.stream(stream):
                 results[index] = davidson(problem.operator, problem.diagonal,
                                           problem.number_of_roots,
                                           initial_vectors=problem.initial_vectors,
-                                          operator_into=problem.operator_into, **kwargs)
+                                          operator_into=problem.operator_into,
+                                          **_kwargs(problem, common))
                 stream.synchronize()
         finally:
             slots.append(slot)

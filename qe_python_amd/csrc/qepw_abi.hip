@@ -336,6 +336,31 @@ int qepw_apply_local(const qepw_plan* plan, int precision, const void* vectors, 
   return finish(rc, "qepw_apply_local");
 }
 
+int qepw_apply_local_group(int count, const qepw_plan* const* plans, int precision,
+                           const void* const* vectors, const int64_t* ldv, const int* nb,
+                           const void* const* potentials, const void* const* kinetics,
+                           void* const* results, const int64_t* ldr, void* const* workspaces,
+                           const size_t* ws_bytes, void* stream) {
+  if (count < 0 || (count > 0 && (!plans || !vectors || !ldv || !nb || !potentials ||
+                                  !kinetics || !results || !ldr || !workspaces || !ws_bytes)))
+    return fail(QEPW_ERR_ARG, "qepw_apply_local_group: NULL argument");
+  if (precision != QEPW_C128 && precision != QEPW_C64)
+    return fail(QEPW_ERR_ARG, "qepw_apply_local_group: unknown precision");
+  std::vector<const qepw::Plan*> ps((size_t)count);
+  for (int i = 0; i < count; ++i) {
+    if (!plans[i]) return fail(QEPW_ERR_ARG, "qepw_apply_local_group: NULL plan");
+    if (nb[i] < 0) return fail(QEPW_ERR_ARG, "qepw_apply_local_group: negative band count");
+    if (nb[i] > 0 && (!vectors[i] || !potentials[i] || !results[i] || !workspaces[i] ||
+                      ldv[i] < plans[i]->p.npw || ldr[i] < plans[i]->p.npw))
+      return fail(QEPW_ERR_ARG, "qepw_apply_local_group: bad problem arguments");
+    ps[(size_t)i] = &plans[i]->p;
+  }
+  return finish(qepw::apply_local_group(count, ps.data(), precision, vectors, ldv, nb,
+                                        potentials, kinetics, results, ldr, workspaces,
+                                        ws_bytes, static_cast<hipStream_t>(stream)),
+                "qepw_apply_local_group");
+}
+
 size_t qepw_inverse_real_workspace(const qepw_plan* plan) {
   return plan ? qepw::inverse_real_bytes(plan->p) : 0;
 }

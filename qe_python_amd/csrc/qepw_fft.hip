@@ -16,6 +16,9 @@
 //   C  k_sticks_forward : forward z-FFT, gather the sphere rows, scale 1/N and
 //                         add T_G psi_G.
 // Only active sticks (19% of the 72^3 grid at 30 Ry) travel through HBM.
+#include <algorithm>
+#include <vector>
+
 #include "qepw_internal.h"
 #include "qepw_fft_lds.h"
 #include "qepw_regfft.h"
@@ -334,8 +337,8 @@ template <int N> struct TeamGeom {
 };
 
 template <typename T, int N, bool PRUNE>
-__global__ __launch_bounds__(kTeamThreads) void k_planes_team(
-    cplx<T>* __restrict__ work, int64_t band_stride, int nst,
+__device__ __forceinline__ void planes_team_body(
+    int z, int64_t b, cplx<T>* __restrict__ work, int64_t band_stride, int nst,
     const int* __restrict__ stick_q, const int* __restrict__ colmap, int nxc,
     const T* __restrict__ V, const cplx<T>* __restrict__ tw) {
   using G = TeamGeom<N>;
@@ -356,8 +359,6 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
   const bool in_team = team_in_wave < G::per_wave;
   const int team = wave * G::per_wave + team_in_wave;
   cplx<T>* S = scratch_all + team * NP;
-  const int z = blockIdx.x;
-  const int64_t b = blockIdx.y;
   const int nq = (nxc + 1) * NP;
   const cplx<T> zero = mk<T>(T(0), T(0));
   cplx<T>* w = work + b * band_stride + (int64_t)z * nst;
@@ -465,6 +466,15 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_team(
   }
 }
 
+template <typename T, int N, bool PRUNE>
+__global__ __launch_bounds__(kTeamThreads) void k_planes_team(
+    cplx<T>* __restrict__ work, int64_t band_stride, int nst,
+    const int* __restrict__ stick_q, const int* __restrict__ colmap, int nxc,
+    const T* __restrict__ V, const cplx<T>* __restrict__ tw) {
+  planes_team_body<T, N, PRUNE>(blockIdx.x, blockIdx.y, work, band_stride, nst, stick_q,
+                                colmap, nxc, V, tw);
+}
+
 // Slots of the team FFT of length n that hold any active x column or stick y
 // (bit s: positions t + M s, t < M); 0 when n has no team FFT.
 unsigned team_slot_mask(int n, const std::vector<int>& xs, const std::vector<int>& ys) {
@@ -482,9 +492,9 @@ unsigned team_slot_mask(int n, const std::vector<int>& xs, const std::vector<int
 // in LDS doubling as the team's exchange buffer; two workgroup barriers in
 // all instead of two per radix pass.
 template <typename T, int N>
-__global__ __launch_bounds__(kTeamThreads) void k_sticks_inverse_team(
-    const cplx<T>* __restrict__ psi, int64_t ldpsi, cplx<T>* __restrict__ work,
-    int64_t band_stride, int nst, const int* __restrict__ off,
+__device__ __forceinline__ void sticks_inverse_body(
+    int gx, int64_t b, const cplx<T>* __restrict__ psi, int64_t ldpsi,
+    cplx<T>* __restrict__ work, int64_t band_stride, int nst, const int* __restrict__ off,
     const int* __restrict__ gpos, const cplx<T>* __restrict__ tw) {
   using G = TeamGeom<N>;
   constexpr int NP = N + 1;
@@ -495,9 +505,8 @@ __global__ __launch_bounds__(kTeamThreads) void k_sticks_inverse_team(
   const int team_in_wave = lane / G::M;
   const int t = lane - team_in_wave * G::M;
   const int team = wave * G::per_wave + team_in_wave;
-  const int s0 = blockIdx.x * G::teams;
+  const int s0 = gx * G::teams;
   const int ns = min(G::teams, nst - s0);
-  const int64_t b = blockIdx.y;
   const cplx<T> zero = mk<T>(T(0), T(0));
   TeamTwiddles<T, N> twr;
   twr.load(t, tw);
@@ -523,11 +532,20 @@ __global__ __launch_bounds__(kTeamThreads) void k_sticks_inverse_team(
 }
 
 template <typename T, int N>
-__global__ __launch_bounds__(kTeamThreads) void k_sticks_forward_team(
-    const cplx<T>* __restrict__ work, int64_t band_stride, const cplx<T>* __restrict__ psi,
-    int64_t ldpsi, const T* __restrict__ kinetic, cplx<T>* __restrict__ out, int64_t ldout,
-    int nst, const int* __restrict__ off, const int* __restrict__ gpos,
-    const cplx<T>* __restrict__ tw, T inv_n) {
+__global__ __launch_bounds__(kTeamThreads) void k_sticks_inverse_team(
+    const cplx<T>* __restrict__ psi, int64_t ldpsi, cplx<T>* __restrict__ work,
+    int64_t band_stride, int nst, const int* __restrict__ off,
+    const int* __restrict__ gpos, const cplx<T>* __restrict__ tw) {
+  sticks_inverse_body<T, N>(blockIdx.x, blockIdx.y, psi, ldpsi, work, band_stride, nst, off,
+                            gpos, tw);
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void sticks_forward_body(
+    int gx, int64_t b, const cplx<T>* __restrict__ work, int64_t band_stride,
+    const cplx<T>* __restrict__ psi, int64_t ldpsi, const T* __restrict__ kinetic,
+    cplx<T>* __restrict__ out, int64_t ldout, int nst, const int* __restrict__ off,
+    const int* __restrict__ gpos, const cplx<T>* __restrict__ tw, T inv_n) {
   using G = TeamGeom<N>;
   constexpr int NP = N + 1;
   extern __shared__ unsigned char smem_raw[];
@@ -537,9 +555,8 @@ __global__ __launch_bounds__(kTeamThreads) void k_sticks_forward_team(
   const int team_in_wave = lane / G::M;
   const int t = lane - team_in_wave * G::M;
   const int team = wave * G::per_wave + team_in_wave;
-  const int s0 = blockIdx.x * G::teams;
+  const int s0 = gx * G::teams;
   const int ns = min(G::teams, nst - s0);
-  const int64_t b = blockIdx.y;
   TeamTwiddles<T, N> twr;
   twr.load(t, tw);
   const cplx<T>* w = work + b * band_stride + s0;
@@ -569,10 +586,136 @@ __global__ __launch_bounds__(kTeamThreads) void k_sticks_forward_team(
   }
 }
 
+template <typename T, int N>
+__global__ __launch_bounds__(kTeamThreads) void k_sticks_forward_team(
+    const cplx<T>* __restrict__ work, int64_t band_stride, const cplx<T>* __restrict__ psi,
+    int64_t ldpsi, const T* __restrict__ kinetic, cplx<T>* __restrict__ out, int64_t ldout,
+    int nst, const int* __restrict__ off, const int* __restrict__ gpos,
+    const cplx<T>* __restrict__ tw, T inv_n) {
+  sticks_forward_body<T, N>(blockIdx.x, blockIdx.y, work, band_stride, psi, ldpsi, kinetic,
+                            out, ldout, nst, off, gpos, tw, inv_n);
+}
+
+// ---- one launch per stage for many problems --------------------------------
+// The h_psi FFT phase of up to kMaxFftGroup independent problems (k-points,
+// cells: each its own plan, bands, potential and workspace, in one precision)
+// runs as three launches in all.  Every problem's workgroups of a stage are
+// laid end to end along x (start[p] .. start[p + 1]); a workgroup finds its
+// problem from that prefix, then runs the same register team FFT body as the
+// one-problem kernels for its grid length, chosen at run time among
+// QEPW_GROUP_SIZES.
+#define QEPW_GROUP_SIZES(X) X(36) X(48) X(54) X(60) X(64) X(72) X(80) X(96)
+constexpr int kMaxFftGroup = 24;
+
+template <typename T> struct StickGroupArgs {
+  int count;
+  int start[kMaxFftGroup + 1];
+  int groups[kMaxFftGroup];        // workgroups per band
+  int n[kMaxFftGroup];             // z length
+  int nst[kMaxFftGroup];
+  const cplx<T>* psi[kMaxFftGroup];
+  int64_t ldpsi[kMaxFftGroup];
+  cplx<T>* work[kMaxFftGroup];
+  int64_t bstride[kMaxFftGroup];
+  const int* off[kMaxFftGroup];
+  const int* gpos[kMaxFftGroup];
+  const cplx<T>* tw[kMaxFftGroup];
+  const T* kin[kMaxFftGroup];       // forward stage
+  cplx<T>* out[kMaxFftGroup];
+  int64_t ldout[kMaxFftGroup];
+  T inv_n[kMaxFftGroup];
+};
+
+template <typename T> struct PlaneGroupArgs {
+  int count;
+  int start[kMaxFftGroup + 1];
+  int n[kMaxFftGroup];             // plane side (n1 = n2)
+  int n3[kMaxFftGroup];
+  int prune[kMaxFftGroup];
+  int nst[kMaxFftGroup];
+  int nxc[kMaxFftGroup];
+  cplx<T>* work[kMaxFftGroup];
+  int64_t bstride[kMaxFftGroup];
+  const int* stickq[kMaxFftGroup];
+  const int* colmap[kMaxFftGroup];
+  const T* V[kMaxFftGroup];
+  const cplx<T>* tw[kMaxFftGroup];
+};
+
+__device__ __forceinline__ int group_of(const int* start, int count, int bx) {
+  int p = 0;
+  while (p + 1 < count && start[p + 1] <= bx) ++p;
+  return p;
+}
+
+template <typename T, bool FORWARD>
+__global__ __launch_bounds__(kTeamThreads) void k_sticks_group(const StickGroupArgs<T> g) {
+  const int p = group_of(g.start, g.count, blockIdx.x);
+  const int l = blockIdx.x - g.start[p];
+  const int gx = l % g.groups[p];
+  const int64_t b = l / g.groups[p];
+  switch (g.n[p]) {
+#define QEPW_STICK_CASE(NN)                                                                     \
+  case NN:                                                                                      \
+    if (FORWARD)                                                                                \
+      sticks_forward_body<T, NN>(gx, b, g.work[p], g.bstride[p], g.psi[p], g.ldpsi[p],          \
+                                 g.kin[p], g.out[p], g.ldout[p], g.nst[p], g.off[p], g.gpos[p], \
+                                 g.tw[p], g.inv_n[p]);                                          \
+    else                                                                                        \
+      sticks_inverse_body<T, NN>(gx, b, g.psi[p], g.ldpsi[p], g.work[p], g.bstride[p],          \
+                                 g.nst[p], g.off[p], g.gpos[p], g.tw[p]);                       \
+    break;
+    QEPW_GROUP_SIZES(QEPW_STICK_CASE)
+#undef QEPW_STICK_CASE
+    default: break;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTeamThreads) void k_planes_group(const PlaneGroupArgs<T> g) {
+  const int p = group_of(g.start, g.count, blockIdx.x);
+  const int l = blockIdx.x - g.start[p];
+  const int z = l % g.n3[p];
+  const int64_t b = l / g.n3[p];
+  switch (g.n[p]) {
+#define QEPW_PLANE_CASE(NN)                                                                     \
+  case NN:                                                                                      \
+    if (g.prune[p])                                                                             \
+      planes_team_body<T, NN, true>(z, b, g.work[p], g.bstride[p], g.nst[p], g.stickq[p],       \
+                                    g.colmap[p], g.nxc[p], g.V[p], g.tw[p]);                    \
+    else                                                                                        \
+      planes_team_body<T, NN, false>(z, b, g.work[p], g.bstride[p], g.nst[p], g.stickq[p],      \
+                                     g.colmap[p], g.nxc[p], g.V[p], g.tw[p]);                   \
+    break;
+    QEPW_GROUP_SIZES(QEPW_PLANE_CASE)
+#undef QEPW_PLANE_CASE
+    default: break;
+  }
+}
+
 // Square plane sizes with an instantiated fast kernel.
 #define QEPW_TEAM_SIZES(X) \
   X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(45) X(48) X(50) X(54) \
   X(60) X(64) X(72) X(75) X(80) X(81) X(90) X(96) X(100) X(108) X(120)
+
+// Teams per workgroup of the team FFT of length n (0: no team kernel).
+static int team_teams(int n) {
+  int teams = 0;
+#define QEPW_TEAMS_N(NN) if (n == NN) teams = TeamGeom<NN>::teams;
+  QEPW_TEAM_SIZES(QEPW_TEAMS_N)
+#undef QEPW_TEAMS_N
+  return teams;
+}
+
+// Whether the pruned plane kernel (only the first two and last two slots
+// occupied) serves a plan whose occupied slots are ``mask``.
+static bool team_prunable(int n, unsigned mask) {
+  bool ok = false;
+#define QEPW_PRUNE_N(NN) if (n == NN) ok = (mask & ~TeamGeom<NN>::pruned) == 0u;
+  QEPW_TEAM_SIZES(QEPW_PRUNE_N)
+#undef QEPW_PRUNE_N
+  return ok;
+}
 
 template <typename T>
 static size_t team_lds_bytes(int n, int nxc) {
@@ -779,6 +922,120 @@ int apply_local(const Plan& p, int precision, const void* vectors, int64_t ldv, 
   if (precision == QEPW_C64)
     return apply_local_t<float>(p, vectors, ldv, nb, potential, kinetic, result, ldr,
                                 workspace, ws_bytes, stream);
+  return QEPW_ERR_ARG;
+}
+
+static bool group_size(int n) {
+#define QEPW_GROUP_IS(NN) if (n == NN) return true;
+  QEPW_GROUP_SIZES(QEPW_GROUP_IS)
+#undef QEPW_GROUP_IS
+  return false;
+}
+
+// Whether a problem's FFT phase can join a grouped launch: square planes and
+// a z length among QEPW_GROUP_SIZES, the team plane path, its whole band
+// block in one workspace chunk.
+template <typename T>
+static bool group_eligible(const Plan& p, int nb, size_t ws_bytes) {
+  const StickTables& st = p.sphere;
+  if (!(p.n1 == p.n2 && group_size(p.n1) && group_size(p.n3))) return false;
+  if (plane_path<T>(p, st) != kPathTeam || !team_available<T>(p, st)) return false;
+  return (size_t)nb * band_bytes_t<T>(p, st) <= ws_bytes && nb <= 65535;
+}
+
+template <typename T>
+int apply_local_group_t(int count, const Plan* const* plans, const void* const* vectors,
+                        const int64_t* ldv, const int* nb, const void* const* potentials,
+                        const void* const* kinetics, void* const* results, const int64_t* ldr,
+                        void* const* workspaces, const size_t* ws_bytes, hipStream_t stream) {
+  // problems that cannot join run on their own; the rest in groups of at most
+  // kMaxFftGroup, three launches per group
+  std::vector<int> members;
+  for (int i = 0; i < count; ++i) {
+    if (nb[i] <= 0) continue;
+    if (group_eligible<T>(*plans[i], nb[i], ws_bytes[i])) {
+      members.push_back(i);
+    } else {
+      const int rc = apply_local_t<T>(*plans[i], vectors[i], ldv[i], nb[i], potentials[i],
+                                      kinetics[i], results[i], ldr[i], workspaces[i],
+                                      ws_bytes[i], stream);
+      if (rc != QEPW_OK) return rc;
+    }
+  }
+  for (size_t g0 = 0; g0 < members.size(); g0 += kMaxFftGroup) {
+    const int m = (int)std::min<size_t>(kMaxFftGroup, members.size() - g0);
+    StickGroupArgs<T> sa{};
+    PlaneGroupArgs<T> pa{};
+    sa.count = pa.count = m;
+    size_t stick_lds = 0, plane_lds = 0;
+    int64_t stotal = 0, ptotal = 0;
+    for (int j = 0; j < m; ++j) {
+      const int i = members[g0 + j];
+      const Plan& p = *plans[i];
+      const StickTables& st = p.sphere;
+      const int teams_z = team_teams(p.n3);
+      sa.start[j] = (int)stotal;
+      sa.groups[j] = (st.nst + teams_z - 1) / teams_z;
+      sa.n[j] = p.n3;
+      sa.nst[j] = st.nst;
+      sa.psi[j] = static_cast<const cplx<T>*>(vectors[i]);
+      sa.ldpsi[j] = ldv[i];
+      sa.work[j] = static_cast<cplx<T>*>(workspaces[i]);
+      sa.bstride[j] = (int64_t)p.n3 * st.nst;
+      sa.off[j] = st.d_off;
+      sa.gpos[j] = st.d_gpos;
+      sa.tw[j] = twiddles<T>(p, 2);
+      sa.kin[j] = static_cast<const T*>(kinetics[i]);
+      sa.out[j] = static_cast<cplx<T>*>(results[i]);
+      sa.ldout[j] = ldr[i];
+      sa.inv_n[j] = T(1.0 / ((double)p.n1 * p.n2 * p.n3));
+      stotal += (int64_t)sa.groups[j] * nb[i];
+      stick_lds = std::max(stick_lds, (size_t)teams_z * (p.n3 + 1) * sizeof(cplx<T>));
+      pa.start[j] = (int)ptotal;
+      pa.n[j] = p.n1;
+      pa.n3[j] = p.n3;
+      pa.prune[j] = team_prunable(p.n1, st.slot_mask) ? 1 : 0;
+      pa.nst[j] = st.nst;
+      pa.nxc[j] = st.nxcols;
+      pa.work[j] = sa.work[j];
+      pa.bstride[j] = sa.bstride[j];
+      pa.stickq[j] = st.d_stickq;
+      pa.colmap[j] = st.d_colmap;
+      pa.V[j] = static_cast<const T*>(potentials[i]);
+      pa.tw[j] = twiddles<T>(p, 0);
+      ptotal += (int64_t)p.n3 * nb[i];
+      plane_lds = std::max(plane_lds, team_lds_bytes<T>(p.n1, st.nxcols));
+    }
+    sa.start[m] = (int)stotal;
+    pa.start[m] = (int)ptotal;
+    if (stotal > 0x7fffffff || ptotal > 0x7fffffff) return QEPW_ERR_UNSUPPORTED;
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sticks_group<T, false>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)stick_lds);
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sticks_group<T, true>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)stick_lds);
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_planes_group<T>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_lds);
+    hipLaunchKernelGGL((k_sticks_group<T, false>), dim3((unsigned)stotal), dim3(kTeamThreads),
+                       stick_lds, stream, sa);
+    hipLaunchKernelGGL(k_planes_group<T>, dim3((unsigned)ptotal), dim3(kTeamThreads),
+                       plane_lds, stream, pa);
+    hipLaunchKernelGGL((k_sticks_group<T, true>), dim3((unsigned)stotal), dim3(kTeamThreads),
+                       stick_lds, stream, sa);
+  }
+  return hipGetLastError() == hipSuccess ? QEPW_OK : QEPW_ERR_HIP;
+}
+
+int apply_local_group(int count, const Plan* const* plans, int precision,
+                      const void* const* vectors, const int64_t* ldv, const int* nb,
+                      const void* const* potentials, const void* const* kinetics,
+                      void* const* results, const int64_t* ldr, void* const* workspaces,
+                      const size_t* ws_bytes, hipStream_t stream) {
+  if (precision == QEPW_C128)
+    return apply_local_group_t<double>(count, plans, vectors, ldv, nb, potentials, kinetics,
+                                       results, ldr, workspaces, ws_bytes, stream);
+  if (precision == QEPW_C64)
+    return apply_local_group_t<float>(count, plans, vectors, ldv, nb, potentials, kinetics,
+                                      results, ldr, workspaces, ws_bytes, stream);
   return QEPW_ERR_ARG;
 }
 

@@ -12,6 +12,11 @@ constexpr int kMaxLdsBytes = 160 * 1024;
 // qepw_fft.hip
 int stick_group_for(int n3, int nst);
 size_t apply_local_band_bytes(const Plan& p, int precision);
+int apply_local_group(int count, const Plan* const* plans, int precision,
+                      const void* const* vectors, const int64_t* ldv, const int* nb,
+                      const void* const* potentials, const void* const* kinetics,
+                      void* const* results, const int64_t* ldr, void* const* workspaces,
+                      const size_t* ws_bytes, hipStream_t stream);
 int apply_local(const Plan& p, int precision, const void* vectors, int64_t ldv, int nb,
                 const void* potential, const void* kinetic, void* result, int64_t ldr,
                 void* workspace, size_t ws_bytes, hipStream_t stream);

@@ -418,6 +418,43 @@ class DeviceFFTPlan:
             buf.data_ptr(), min(buf.numel(), need), stream_handle(vectors.device))
         nat.check(rc, "qepw_apply_local")
 
+    @staticmethod
+    def apply_local_group(items) -> None:
+        """``apply_local`` of several plans at once, ``items`` a sequence of
+        (plan, vectors, result, potential, kinetic, workspace) in one
+        precision: grouped launches of the FFT stages (qepw_apply_local_group);
+        each plan's workspace holds all its bands."""
+        items = [it for it in items if it[1].shape[0] > 0]
+        if not items:
+            return
+        n = len(items)
+        dtype = items[0][1].dtype
+        prec = precision_code(dtype)
+        plans, vecs, ldv, nbs, pots, kins, outs, ldr, bufs, sizes = ([] for _ in range(10))
+        for plan, vectors, result, potential, kinetic, ws in items:
+            if (vectors.dtype != dtype or result.dtype != dtype
+                    or potential.dtype != real_dtype(dtype)):
+                raise ValueError("local-potential operands disagree in precision")
+            nb = int(vectors.shape[0])
+            need = plan.workspace_bytes(dtype, nb)
+            buf = ws.get(need)
+            plans.append(plan.handle)
+            vecs.append(vectors.data_ptr())
+            ldv.append(ld_of(vectors))
+            nbs.append(nb)
+            pots.append(potential.data_ptr())
+            kins.append(None if kinetic is None else kinetic.data_ptr())
+            outs.append(result.data_ptr())
+            ldr.append(ld_of(result))
+            bufs.append(buf.data_ptr())
+            sizes.append(min(buf.numel(), need))
+        P = ctypes.c_void_p * n
+        rc = nat.load().qepw_apply_local_group(
+            n, P(*plans), prec, P(*vecs), (ctypes.c_int64 * n)(*ldv), (ctypes.c_int * n)(*nbs),
+            P(*pots), P(*kins), P(*outs), (ctypes.c_int64 * n)(*ldr), P(*bufs),
+            (ctypes.c_size_t * n)(*sizes), stream_handle(items[0][1].device))
+        nat.check(rc, "qepw_apply_local_group")
+
     def inverse_real(self, potential_g: torch.Tensor, ws: Workspace) -> torch.Tensor:
         """(n3, n2, n1) float64 = Re backward FFT of a C-order complex grid."""
         if potential_g.dtype != C128 or tuple(potential_g.shape) != self.shape:

@@ -49,6 +49,8 @@ import torch
 from . import _native as nat
 from . import device as dv
 from .eigensolver import DeviceRayleighRitz, _lower_inverse, _one_blas_thread
+from . import schedule
+from .schedule import Apply, Read, blocking
 
 # default reduced eigensolve of ``davidson``: "device" (GPU spectral
 # division) or "host" (LAPACK zhegvd, the reference's routine)
@@ -421,16 +423,41 @@ class PlaneWaveHamiltonian:
                 stop.record()
                 self.fft_events.append((nb, src.dtype, start, stop))
         if res.beta is not None:
-            p = res.nproj
-            with _measure(self.timers, "h_psi:calbec", self.device):
-                becp = torch.empty((nb, p), dtype=src.dtype, device=self.device)
-                dv.gemm(nat.OP_C, p, nb, self.npw, 1.0, res.beta, src, 0.0, becp, self.workspace)
-            with _measure(self.timers, "add_vuspsi", self.device):
-                coupled = torch.empty_like(becp)
-                dv.gemm(nat.OP_N, p, nb, p, 1.0, res.coupling, becp, 0.0, coupled, self.workspace)
-                dv.gemm(nat.OP_N, self.npw, nb, p, 1.0, res.beta, coupled, 1.0, dst,
-                        self.workspace)
+            self._nonlocal_apply(src, dst, res)
         self.applications += nb
+
+    def _nonlocal_apply(self, src: torch.Tensor, dst: torch.Tensor, res) -> None:
+        """dst += sum_terms beta D beta^H src (calbec, add_vuspsi)."""
+        nb = int(src.shape[0])
+        p = res.nproj
+        with _measure(self.timers, "h_psi:calbec", self.device):
+            becp = torch.empty((nb, p), dtype=src.dtype, device=self.device)
+            dv.gemm(nat.OP_C, p, nb, self.npw, 1.0, res.beta, src, 0.0, becp, self.workspace)
+        with _measure(self.timers, "add_vuspsi", self.device):
+            coupled = torch.empty_like(becp)
+            dv.gemm(nat.OP_N, p, nb, p, 1.0, res.coupling, becp, 0.0, coupled, self.workspace)
+            dv.gemm(nat.OP_N, self.npw, nb, p, 1.0, res.beta, coupled, 1.0, dst, self.workspace)
+
+    @staticmethod
+    def device_apply_group(requests) -> None:
+        """``device_apply`` of several Hamiltonians at once: ``requests`` a
+        sequence of (hamiltonian, src, dst) in one precision.  The FFT phase of
+        all of them runs as one grouped launch per stage
+        (``DeviceFFTPlan.apply_local_group``), then each one's projector
+        terms; results equal the one-by-one calls."""
+        requests = [r for r in requests if int(r[1].shape[0]) > 0]
+        if not requests:
+            return
+        items = []
+        for ham, src, dst in requests:
+            res = ham._resources(src.dtype)
+            items.append((ham.plan, src, dst, res.potential, res.kinetic, ham.workspace))
+        dv.DeviceFFTPlan.apply_local_group(items)
+        for ham, src, dst in requests:
+            res = ham._resources(src.dtype)
+            if res.beta is not None:
+                ham._nonlocal_apply(src, dst, res)
+            ham.applications += int(src.shape[0])
 
     def to_device(self, vectors, dtype=torch.complex128) -> torch.Tensor:
         """(npw, nb) reference-order host array or device tensor -> (nb, npw)
@@ -580,7 +607,7 @@ def _host_dtype(dtype: torch.dtype):
 
 
 def _cholesky_orthonormalize(block: torch.Tensor, npw: int, ws, condition: float = 1.0e6):
-    """Cholesky QR of a block of complex64 corrections: Gram = L L^H on the
+    """(steps) Cholesky QR of a block of complex64 corrections: Gram = L L^H on the
     host (one thread), block L^-H on the device.  None when the Gram matrix
     is not safely positive definite at FP32 precision (its Cholesky fails or
     the squared condition estimate max(diag L)^2 / min(diag L)^2 exceeds
@@ -590,7 +617,7 @@ def _cholesky_orthonormalize(block: torch.Tensor, npw: int, ws, condition: float
     device = block.device
     gram_t = torch.empty((n, n), dtype=block.dtype, device=device)
     dv.gemm(nat.OP_C, n, n, npw, 1.0, block, block, 0.0, gram_t, ws)
-    gram = gram_t.cpu().numpy().T.astype(np.complex128)
+    gram = (yield Read(gram_t)).T.astype(np.complex128)
     gram = 0.5 * (gram + gram.conj().T)
     with _one_blas_thread():
         try:
@@ -611,13 +638,18 @@ def _orthonormalize_device(block: torch.Tensor, npw: int, ws, threshold: float =
     """Canonical orthonormalisation (``diagonalization.py:450-472``, serial):
     Gram V^H V on the device, its small Hermitian eigensolve on the host,
     V U s^{-1/2} (columns by decreasing Gram eigenvalue) on the device."""
+    return blocking(_orthonormalize_steps(block, npw, ws, threshold))
+
+
+def _orthonormalize_steps(block: torch.Tensor, npw: int, ws, threshold: float = 1.0e-12):
+    """(steps) ``_orthonormalize_device`` as a generator of reads."""
     n = int(block.shape[0])
     device = block.device
     if n == 0:
         return block[:0].clone()
     if n == 1:
         sq = dv.column_squared_norms(block, npw)
-        if float(sq.cpu()[0]) == 0.0:
+        if float((yield Read(sq))[0]) == 0.0:
             return block[:0].clone()
         out = torch.empty_like(block)
         dv.normalize_selected_columns(block, sq, dv.index_tensor([0], device),
@@ -625,7 +657,7 @@ def _orthonormalize_device(block: torch.Tensor, npw: int, ws, threshold: float =
         return out
     gram_t = torch.empty((n, n), dtype=block.dtype, device=device)
     dv.gemm(nat.OP_C, n, n, npw, 1.0, block, block, 0.0, gram_t, ws)
-    gram = gram_t.cpu().numpy().T.astype(np.complex128)
+    gram = (yield Read(gram_t)).T.astype(np.complex128)
     gram = 0.5 * (gram + gram.conj().T)
     with _one_blas_thread():
         squared, rotation = np.linalg.eigh(gram)
@@ -675,6 +707,36 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
       takes at most ``max_iterations - 1``, the refinement the rest; with
       ``max_iterations < 2`` the FP64 form runs).
     """
+    steps = davidson_steps(
+        operator, diagonal, number_of_roots, initial_vectors=initial_vectors,
+        tolerance=tolerance, max_iterations=max_iterations,
+        subspace_multiplier=subspace_multiplier, residual_factor=residual_factor,
+        residual_energy_scale=residual_energy_scale, occupied_roots=occupied_roots,
+        initial_is_ritz=initial_is_ritz, initial_is_orthonormal=initial_is_orthonormal,
+        initial_eigenvalues=initial_eigenvalues, initial_applied=initial_applied, mpi=mpi,
+        global_dimension=global_dimension, global_row_indices=global_row_indices,
+        timers=timers, operator_into=operator_into, device_output=device_output,
+        reduced_eigensolver=reduced_eigensolver, precision=precision,
+        mixed_switch=mixed_switch)
+    return schedule.run(steps, _solve_device(operator, operator_into))
+
+
+def _solve_device(operator, operator_into) -> torch.device:
+    ham = _device_hamiltonian(operator, operator_into)
+    return ham.device if ham is not None else dv.require_device()
+
+
+def davidson_steps(operator, diagonal, number_of_roots, *, initial_vectors=None, tolerance=1.0e-8,
+             max_iterations=40, subspace_multiplier=4, residual_factor=10.0,
+             residual_energy_scale=None, occupied_roots=None, initial_is_ritz=False,
+             initial_is_orthonormal=False, initial_eigenvalues=None, initial_applied=None,
+             mpi=None, global_dimension=None, global_row_indices=None, timers=None,
+             operator_into=None, device_output=False, reduced_eigensolver=None,
+             precision=None, mixed_switch=None):
+    """(steps) ``davidson`` as a generator of ``qe_python_amd.schedule``
+    requests; its return value is the ``DavidsonResult``.  Drive it with
+    ``schedule.run`` (one solve) or ``schedule.run_many`` (several solves
+    in rounds, H applied to all of them together)."""
     if reduced_eigensolver is None:
         reduced_eigensolver = REDUCED_EIGENSOLVER
     if reduced_eigensolver not in ("device", "host"):
@@ -759,7 +821,7 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
     if precision == "fp64":
         dtype = torch.complex128
         start = _upload_block(trial, perm_dev, npw, device, dtype)
-        phase = _davidson_phase(
+        phase = yield from _davidson_phase_body(
             ctx, start, dtype, energy_bounds, residual_bounds, residual_energy_scale,
             root_tolerances, max_iterations, initial_is_ritz, initial_is_orthonormal,
             initial_eigenvalues, applied_block, perm_dev)
@@ -777,7 +839,7 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
     if MIXED_COARSE_MULTIPLIER is not None:
         coarse_ctx = dataclasses.replace(
             ctx, subspace_multiplier=min(ctx.subspace_multiplier, MIXED_COARSE_MULTIPLIER))
-    coarse = _davidson_phase(
+    coarse = yield from _davidson_phase_body(
         coarse_ctx, start, torch.complex64, np.maximum(energy_bounds, de_switch),
         np.full(number_of_roots, res_switch), None, np.maximum(root_tolerances, de_switch),
         max_iterations - 1, initial_is_ritz, initial_is_orthonormal, initial_eigenvalues,
@@ -789,7 +851,7 @@ def davidson(operator, diagonal, number_of_roots, *, initial_vectors=None, toler
     start = torch.empty(coarse.X.shape, dtype=torch.complex128, device=device)
     dv.copy_columns(coarse.X, start, npw)
     coarse.X = None
-    fine = _davidson_phase(
+    fine = yield from _davidson_phase_body(
         ctx, start, torch.complex128, energy_bounds, residual_bounds, residual_energy_scale,
         root_tolerances, max(1, max_iterations - coarse.iterations), False, True, None,
         lambda active: None, perm_dev, seed_spectrum=coarse.spectrum)
@@ -822,11 +884,6 @@ class _PhaseResult:
     number_unconverged: int
     happ: int
     spectrum: np.ndarray | None = None   # eigenvalue estimates, also above the roots
-
-
-def _davidson_phase(ctx, *args, **kwargs):
-    with dv.fixed_stream(ctx.device):
-        return _davidson_phase_body(ctx, *args, **kwargs)
 
 
 def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, residual_energy_scale,
@@ -885,7 +942,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
         basis = start[:number_of_roots]
     else:
         with _measure(timers, "cegterg:over", device):
-            basis = _orthonormalize_device(start, npw, ws)
+            basis = yield from _orthonormalize_steps(start, npw, ws)
     if basis.shape[0] < number_of_roots:
         raise ValueError("initial Davidson vectors are linearly dependent")
     maximum_subspace = min(ctx.dimension, max(basis.shape[0], 2 * number_of_roots,
@@ -897,7 +954,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
     del basis, start
     given = applied_block(active)
     if given is None:
-        op.device_apply(V[:active], HV[:active])
+        yield Apply(op, V[:active], HV[:active])
         happ = active
     else:
         HV[:active].copy_(_upload_block(given, perm_dev, npw, device, dtype))
@@ -916,7 +973,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
 
     def ritz_solve():
         with _measure(timers, "cdiaghg", device):
-            roots, coef = rr.solve(active)
+            roots, coef = yield from rr.solve_steps(active)
             if coef.dtype != dtype:
                 narrow = torch.empty(coef.shape, dtype=dtype, device=device)
                 dv.copy_columns(coef, narrow, int(coef.shape[1]))
@@ -969,7 +1026,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
     if start_is_ritz:
         if initial_eigenvalues is None:
             ip = dv.column_inner_products(V[:number_of_roots], HV[:number_of_roots], npw)
-            values = ip.cpu().numpy()[0::2].copy()
+            values = (yield Read(ip))[0::2].copy()
         else:
             values = np.asarray(initial_eigenvalues[:number_of_roots], dtype=float)
         X = V[:number_of_roots].clone()
@@ -980,7 +1037,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
         coef = torch.empty((number_of_roots, number_of_roots), dtype=dtype, device=device)
         dv.set_block(coef, number_of_roots, number_of_roots, 1.0)
     else:
-        values, coef = ritz_solve()
+        values, coef = yield from ritz_solve()
         X, R = ritz_block(values, coef)
         if seed_spectrum is not None and rr.theta is not None:
             # estimates from an earlier stage above the roots: the next solve
@@ -989,10 +1046,11 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
             rr.seed(np.concatenate([rr.theta, np.asarray(seed_spectrum)[known:]]))
 
     def residual_norms_of(block):
-        sq = dv.column_squared_norms(block, npw).cpu().numpy()
+        sq = yield Read(dv.column_squared_norms(block, npw))
         return np.sqrt(np.maximum(sq, 0.0))
 
-    residual_norms = residual_norms_of(R) if track else np.empty(number_of_roots)
+    residual_norms = ((yield from residual_norms_of(R)) if track
+                      else np.empty(number_of_roots))
     completed = 0
     number_unconverged = number_of_roots
     unconverged = np.ones(number_of_roots, dtype=bool)
@@ -1033,9 +1091,10 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
                                 lproj, ws)
                         dv.gemm(nat.OP_N, npw, len(cols), nl, -1.0, locked, lproj, 1.0,
                                 corrections, ws)
-                block = _cholesky_orthonormalize(corrections, npw, ws)
+                block = yield from _cholesky_orthonormalize(corrections, npw, ws)
                 if block is None:
-                    block = _orthonormalize_device(corrections, npw, ws, threshold=1.0e-4)
+                    block = yield from _orthonormalize_steps(corrections, npw, ws,
+                                                             threshold=1.0e-4)
             if not block.shape[0]:
                 break
             _check_room(active, int(block.shape[0]), maximum_subspace)
@@ -1047,7 +1106,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
         else:
             # _normalize_columns (diagonalization.py:500-524): no orthogonalisation
             sq_dev = dv.column_squared_norms(corrections, npw)
-            sq = sq_dev.cpu().numpy()
+            sq = yield Read(sq_dev)
             if not len(sq):
                 break
             scale = max(float(np.max(sq)), 1.0e-300)
@@ -1061,12 +1120,12 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
                                           V[old:active], npw)
         del X, R, corrections
         X = None
-        op.device_apply(V[old:active], HV[old:active])
+        yield Apply(op, V[old:active], HV[old:active])
         happ += len(keep)
         update_projected(old)
         coef_prev = coef
         try:
-            values, coef = ritz_solve()
+            values, coef = yield from ritz_solve()
         except np.linalg.LinAlgError:
             if not coarse:
                 raise
@@ -1076,7 +1135,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
         if track:
             X, R = ritz_block(values, coef)
             packed = False
-            residual_norms = residual_norms_of(R)
+            residual_norms = yield from residual_norms_of(R)
         converged = np.abs(values - previous) < eb
         if rb is not None:
             converged &= residual_norms < rb
@@ -1091,7 +1150,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
         if number_unconverged == 0:
             if not track:
                 X, R = ritz_block(values, coef)
-                residual_norms = residual_norms_of(R)
+                residual_norms = yield from residual_norms_of(R)
             est = spectrum()
             values, X, residual_norms = merged(values, X, residual_norms)
             return _PhaseResult(values, X, residual_norms, iteration, True, 0, happ, est)
@@ -1169,7 +1228,7 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
     if X is None or packed:
         X, R = ritz_block(values, coef)
     if not track:
-        residual_norms = residual_norms_of(R)
+        residual_norms = yield from residual_norms_of(R)
     est = spectrum()
     values, X, residual_norms = merged(values, X, residual_norms)
     return _PhaseResult(values, X, residual_norms, completed, False, number_unconverged, happ,

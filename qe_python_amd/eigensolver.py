@@ -48,6 +48,7 @@ from threadpoolctl import ThreadpoolController
 
 from . import _native as nat
 from . import device as dv
+from .schedule import Read, blocking
 
 C128 = torch.complex128
 _THREADPOOLS = None
@@ -68,6 +69,14 @@ def _lower_inverse(lower: np.ndarray) -> np.ndarray:
     if info != 0:
         raise np.linalg.LinAlgError(f"ztrtri failed (info {info})")
     return np.tril(inv)
+
+
+def _cholesky_and_inverse(m: np.ndarray):
+    """Lower Cholesky factor of a Hermitian positive-definite host matrix
+    and its inverse, on one BLAS thread."""
+    with _one_blas_thread():
+        lower = np.linalg.cholesky(m)
+        return lower, _lower_inverse(lower)
 
 
 def _scaled_ns_alphas(lower: float, tol: float = 1e-15, limit: int = 60) -> list[float]:
@@ -147,8 +156,10 @@ class DeviceRayleighRitz:
         """Column-major view of rows r0:r1, columns c0:c1."""
         return mat[c0:c1, r0:r1]
 
-    def host_block(self, mat: torch.Tensor, n: int) -> np.ndarray:
-        return mat[:n, :n].cpu().numpy().T
+    def host_block(self, mat: torch.Tensor, n: int):
+        """(steps) The leading n x n block of a column-major device matrix
+        as a host array."""
+        return (yield Read(mat[:n, :n])).T
 
     def _gemm(self, op, m, n, k, alpha, a, b, beta, c):
         dv.gemm(op, m, n, k, alpha, a, b, beta, c, self.ws)
@@ -209,7 +220,7 @@ class DeviceRayleighRitz:
         nnew = active - first
         st = dv.stream_handle(self.device)
         if first == 0:
-            m = self.host_block(self.S, active)
+            m = yield from self.host_block(self.S, active)
         else:
             # Z = L21^H = W11 S12 ; M = S22 - Z^H Z
             Z = torch.empty((nnew, first), dtype=C128, device=self.device)
@@ -218,13 +229,12 @@ class DeviceRayleighRitz:
             M = torch.empty((nnew, nnew), dtype=C128, device=self.device)
             dv.copy_columns(self.block(self.S, first, active, first, active), M, nnew, nnew)
             self._gemm(nat.OP_C, nnew, nnew, first, -1.0, Z, Z, 1.0, M)
-            m = M.cpu().numpy().T
+            m = (yield Read(M)).T
         m = 0.5 * (m + m.conj().T)
         try:
-            l22 = np.linalg.cholesky(m)
+            _, w22 = _cholesky_and_inverse(m)
         except np.linalg.LinAlgError as exc:
             raise np.linalg.LinAlgError(f"zhegvd failed: overlap not positive definite ({exc})")
-        w22 = _lower_inverse(l22)
         w22_dev = dv.upload(w22.T, self.device)
         dv.copy_columns(w22_dev, self.block(self.W, first, active, first, active), nnew, nnew)
         # the strictly upper part of the new columns stays zero
@@ -252,35 +262,42 @@ class DeviceRayleighRitz:
     # -- step 3/4: eigenpairs --------------------------------------------
     def solve(self, active: int):
         """Lowest nroots pairs: (values host float64, coefficients device
-        (nroots, active) column-major active x nroots)."""
+        (nroots, active) column-major active x nroots), with blocking reads."""
+        return blocking(self.solve_steps(active))
+
+    def solve_steps(self, active: int):
+        """(steps) ``solve`` as a generator of ``Read`` requests
+        (``qe_python_amd.schedule``)."""
         if not self.spectral:
-            values, coef, _ = self._host(active)
+            values, coef, _ = yield from self._host(active)
             self.stats["host"] += 1
             return values, coef
         k = self.nroots
-        with _one_blas_thread():
-            self._standardize(active)
-            out = None
-            if self.theta is not None and active > k + 8:
-                out = self._spectral(active)
-            else:
-                self.stats["skipped"] += 1
-            if out is not None:
-                self.stats["spectral"] += 1
-            else:
-                out = self._dense(active)
-                self.stats["dense"] += 1
+        # the host factorisations run on one BLAS thread each (a limit that
+        # is process-wide: it is never held across a yield, where another
+        # solve may run)
+        yield from self._standardize(active)
+        out = None
+        if self.theta is not None and active > k + 8:
+            out = yield from self._spectral(active)
+        else:
+            self.stats["skipped"] += 1
+        if out is not None:
+            self.stats["spectral"] += 1
+        else:
+            out = yield from self._dense(active)
+            self.stats["dense"] += 1
         values, coef = out
         if self.check:
-            ref, _, _ = self._host(active)
+            ref, _, _ = yield from self._host(active)
             err = float(np.max(np.abs(ref - values)))
             self.stats["max_check_error"] = max(self.stats["max_check_error"], err)
         return values, coef
 
     def _host(self, active):
         """zhegvd of the copied pencil on the host, as the reference does."""
-        h = self.host_block(self.H, active)
-        s = self.host_block(self.S, active)
+        h = yield from self.host_block(self.H, active)
+        s = yield from self.host_block(self.S, active)
         h = np.array(h, dtype=np.complex128, order="F")
         s = np.array(s, dtype=np.complex128, order="F")
         try:
@@ -297,7 +314,7 @@ class DeviceRayleighRitz:
         one BLAS thread, ~2 ms at 150 rows; the first solve of a pencil and
         the fallbacks); C = W^H U on the device."""
         k = self.nroots
-        a = self.host_block(self.A, n)
+        a = yield from self.host_block(self.A, n)
         w, v = self._host_eigh(0.5 * (a + a.conj().T))
         U = dv.upload(np.ascontiguousarray(v[:, :k].T), self.device)
         coef = torch.empty((k, n), dtype=C128, device=self.device)
@@ -313,9 +330,10 @@ class DeviceRayleighRitz:
         """All eigenpairs of a Hermitian host matrix (LAPACK heevd), in
         complex64 when ``host_single`` (the complex64 phase of the mixed form,
         whose vectors are complex64 anyway), returned in FP64."""
-        if not self.host_single:
-            return np.linalg.eigh(a)
-        w, v = sla.eigh(a.astype(np.complex64), driver="evd", check_finite=False)
+        with _one_blas_thread():
+            if not self.host_single:
+                return np.linalg.eigh(a)
+            w, v = sla.eigh(a.astype(np.complex64), driver="evd", check_finite=False)
         return w.astype(np.float64), v.astype(np.complex128)
 
     def _can_keep_shift(self) -> bool:
@@ -400,7 +418,7 @@ class DeviceRayleighRitz:
         # (0, 1] into itself for any schedule, a poor one just converges
         # later): twice the previous solve's value avoids a read-back here
         if self._bound_prev is None:
-            self._bound_prev = float(self.bound[:1].cpu()[0])
+            self._bound_prev = float((yield Read(self.bound[:1]))[0])
             bound_est = self._bound_prev
         else:
             bound_est = 2.0 * self._bound_prev
@@ -449,7 +467,7 @@ class DeviceRayleighRitz:
             return Q0, self.rb[:h + 2 * count * count]
 
         def unpack(rb, count):
-            host = rb.cpu().numpy()
+            host = yield Read(rb)
             g = host[self._head:].view(np.complex128).reshape(count, count).T
             return (float(host[0]), float(host[1]), float(host[self.maxsub + 2]), g)
 
@@ -459,12 +477,12 @@ class DeviceRayleighRitz:
         sign_steps(alphas)
         count = min(n, len(theta) + 24)
         Q0, rb = sampled_range(count)
-        res2, trace, self._bound_prev, g = unpack(rb, count)
+        res2, trace, self._bound_prev, g = yield from unpack(rb, count)
         attempt = 0
         self._slow_shift = res2 >= 1e-20 * n
         while res2 >= 1e-20 * n and attempt < 2:
             sign_steps([1.0] * 12)
-            res2, trace = (float(v) for v in self.sig[:2].cpu().numpy())
+            res2, trace = (float(v) for v in (yield Read(self.sig[:2])))
             attempt += 1
             g = None
         if res2 >= 1e-20 * n:
@@ -477,7 +495,7 @@ class DeviceRayleighRitz:
         if g is None or m_below > count:
             count = max(count, m_below)
             Q0, rb = sampled_range(count)
-            _, _, _, g = unpack(rb, count)
+            _, _, _, g = yield from unpack(rb, count)
         mp = m_below
         Q0 = Q0[:mp]
         g = g[:mp, :mp]
@@ -486,11 +504,11 @@ class DeviceRayleighRitz:
         # eigenvectors, then Gaussian columns); its Gram matrix is factored by
         # Cholesky and Q = Q0 L^-H is an orthonormal basis of range(P)
         try:
-            lg = np.linalg.cholesky(g)
+            _, lg_inv = _cholesky_and_inverse(g)
         except np.linalg.LinAlgError:
             self.stats["gram_not_pd"] += 1
             return None
-        t = _lower_inverse(lg).conj().T
+        t = lg_inv.conj().T
         t_dev = dv.upload(t.T, self.device)
         Q = torch.empty_like(Q0)
         self._gemm(nat.OP_N, n, mp, mp, 1.0, Q0, t_dev, 0.0, Q)
@@ -498,7 +516,7 @@ class DeviceRayleighRitz:
         self._gemm(nat.OP_N, n, mp, n, 1.0, A, Q, 0.0, AQ)
         B = torch.empty((mp, mp), dtype=C128, device=self.device)
         self._gemm(nat.OP_C, mp, mp, n, 1.0, Q, AQ, 0.0, B)
-        bt = B.cpu().numpy().T
+        bt = (yield Read(B)).T
         bt = 0.5 * (bt + bt.conj().T)
         w, y = self._host_eigh(bt)
         if len(w) < k:

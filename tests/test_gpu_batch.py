@@ -5,7 +5,8 @@ All 24 problems of the uneven batch -- 8 seeded k-points of each of the
 together in the mixed form.  Bars: every eigenvalue within 1e-8 Ry (5e-9 Ha)
 of the oracle eigenvalues (tests/golden/batch_davidson.npz, each bounded to
 far below that by its residuals), and results bit-identical to solving the
-same problems one by one.
+same problems one by one, in both schedules (lock-step rounds with grouped
+H applications, and one thread and stream per problem).
 """
 
 from pathlib import Path
@@ -49,23 +50,29 @@ def batch_problems(systems):
             for _, s, h in systems]
 
 
-def test_batch_matches_oracle_and_sequential(systems):
-    from qe_python_amd.batch import davidson_batch
+@pytest.fixture(scope="module")
+def sequential(systems):
     from qe_python_amd.diagonalization import davidson
+    return [davidson(h.apply, h.diagonal, s.nroots, initial_vectors=s.trial,
+                     operator_into=h.apply_into, **KWARGS) for _, s, h in systems]
+
+
+@pytest.mark.parametrize("schedule", ["rounds", "threads"])
+def test_batch_matches_oracle_and_sequential(systems, sequential, schedule):
+    from qe_python_amd.batch import davidson_batch
     golden = np.load(GOLDEN)
-    results = davidson_batch(batch_problems(systems), workers=4, **KWARGS)
+    results = davidson_batch(batch_problems(systems), workers=4, schedule=schedule, **KWARGS)
     for (name, s, h), res in zip(systems, results):
         assert int(golden[f"{name}_npw"]) == s.npw
         err = np.abs(res.eigenvalues - golden[name])
-        print(f"{name}: npw {s.npw}, {s.nroots} roots, iterations {res.iterations}, "
+        print(f"{schedule} {name}: npw {s.npw}, {s.nroots} roots, iterations {res.iterations}, "
               f"max |de| {err.max():.2e} Ha")
         assert res.converged
         assert np.all(err < TARGET_HA), (name, np.sort(err)[-4:])
-    for (name, s, h), res in zip(systems, results):
-        one = davidson(h.apply, h.diagonal, s.nroots, initial_vectors=s.trial,
-                       operator_into=h.apply_into, **KWARGS)
+    for (name, s, h), res, one in zip(systems, results, sequential):
         np.testing.assert_array_equal(one.eigenvalues, res.eigenvalues, err_msg=name)
         assert one.iterations == res.iterations
+        assert one.hamiltonian_applications == res.hamiltonian_applications
         np.testing.assert_array_equal(one.eigenvectors, res.eigenvectors, err_msg=name)
 
 
@@ -79,3 +86,6 @@ def test_batch_device_output_and_errors(systems):
         davidson_batch([BatchProblem(h.apply, h.diagonal, 4, s.trial, h.apply_into)] * 2,
                        **KWARGS)
     assert davidson_batch([]) == []
+    with pytest.raises(ValueError, match="schedule"):
+        davidson_batch([], schedule="parallel")
+

@@ -258,3 +258,44 @@ def test_shape_errors_are_value_errors(small):
         ham.apply(np.zeros((small.npw + 1, 2), dtype=complex))
     with pytest.raises(ValueError):
         ham.apply_into(small.trial[:, :2], np.zeros((small.npw, 3), dtype=complex))
+
+
+def test_grouped_apply_equals_single_apply():
+    """``PlaneWaveHamiltonian.device_apply_group`` (one grouped launch per
+    FFT stage over several problems of different grids, npw and band counts)
+    gives bit for bit what ``device_apply`` gives per problem, in both
+    precisions; problems whose grid cannot join the group (generic and split
+    plane stages) run on their own inside the same call."""
+    from qe_python_amd.basis import make_bases
+    from qe_python_amd.diagonalization import PlaneWaveHamiltonian
+    rng = np.random.default_rng(41)
+    hams = [device_ham(make_system("small"))]
+    for name, k in (("cell54", [0.21, 0.07, 0.33]), ("cell64", [0.0, 0.5, 0.25]),
+                    ("cell54", [0.4, 0.1, 0.0])):
+        hams.append(device_ham(make_system(name, kpoint=np.array(k))))
+    reciprocal = 2.0 * np.pi / 12.0 * np.eye(3)
+    for shape, ecut in (((30, 30, 30), 10.0), ((15, 12, 12), 1.0), ((128, 128, 20), 8.0)):
+        basis = make_bases(reciprocal, np.array([0.1, 0.3, 0.2]), ecut)[0]
+        hams.append(PlaneWaveHamiltonian(basis, np.zeros(shape, complex), (),
+                                         real_potential=rng.uniform(-1.0, 0.5, shape)))
+    bands = [5, 64, 17, 1, 33, 7, 3, 2]
+    for dtype in (torch.complex128, torch.complex64):
+        requests, singles = [], []
+        for ham, nb in zip(hams, bands):
+            x = rng.standard_normal((ham.npw, nb)) + 1j * rng.standard_normal((ham.npw, nb))
+            src = ham.to_device(x, dtype)
+            one = torch.empty_like(src)
+            # NaN bytes in the scratch: a kernel reading what no stage wrote
+            # would show up in the result
+            ham.workspace.get(ham.plan.workspace_bytes(dtype, nb)).fill_(255)
+            ham.device_apply(src, one)
+            singles.append(one)
+            requests.append((ham, src, torch.full_like(src, float("nan"))))
+            ham.workspace.get(ham.plan.workspace_bytes(dtype, nb)).fill_(255)
+        before = [h.applications for h in hams]
+        PlaneWaveHamiltonian.device_apply_group(requests)
+        for (ham, src, got), one, b0, nb in zip(requests, singles, before, bands):
+            assert bool(torch.isfinite(torch.view_as_real(one)).all()), (ham.plan.shape, dtype)
+            assert torch.equal(got, one), (ham.plan.shape, dtype)
+            assert ham.applications == b0 + nb
+    PlaneWaveHamiltonian.device_apply_group([])
