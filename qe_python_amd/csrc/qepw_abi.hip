@@ -81,8 +81,9 @@ void free_tables(qepw::StickTables& t) {
   hipFree(t.d_stickq);
   hipFree(t.d_colmap);
   hipFree(t.d_colstart);
+  hipFree(t.d_colpos);
   t.d_off = t.d_gpos = t.d_plane = t.d_xcols = t.d_stickq = t.d_colmap = nullptr;
-  t.d_colstart = nullptr;
+  t.d_colstart = t.d_colpos = nullptr;
 }
 
 void free_plan(qepw::Plan& p) {
@@ -99,7 +100,7 @@ void free_plan(qepw::Plan& p) {
 }
 
 struct TableHost {
-  std::vector<int> off, gpos, plane, xcols, stickq, colmap, colstart;
+  std::vector<int> off, gpos, plane, xcols, stickq, colmap, colstart, colpos;
 };
 
 hipError_t upload_tables(qepw::StickTables& t, const TableHost& h) {
@@ -110,6 +111,7 @@ hipError_t upload_tables(qepw::StickTables& t, const TableHost& h) {
   if ((e = upload(&t.d_stickq, h.stickq)) != hipSuccess) return e;
   if ((e = upload(&t.d_colmap, h.colmap)) != hipSuccess) return e;
   if ((e = upload(&t.d_colstart, h.colstart)) != hipSuccess) return e;
+  if ((e = upload(&t.d_colpos, h.colpos)) != hipSuccess) return e;
   return upload(&t.d_xcols, h.xcols);
 }
 
@@ -185,6 +187,8 @@ int qepw_plan_create(const int32_t* miller, int64_t npw, int n1, int n2, int n3,
     sphere.stickq[s] = sphere.colmap[x] * n2 + y;
     sphere.colstart[sphere.colmap[x]] = s;  // sticks are sorted by x, then y
   }
+  sphere.colpos.assign(sphere.xcols.size() * (size_t)n2, -1);
+  for (int s = 0; s < nst; ++s) sphere.colpos[sphere.stickq[s]] = s;
 
   TableHost full;
   const int nfull = n1 * n2;
@@ -199,6 +203,8 @@ int qepw_plan_create(const int32_t* miller, int64_t npw, int n1, int n2, int n3,
   for (int s = 0; s < nfull; ++s) full.stickq[s] = s;  // column x = s / n2, y = s % n2
   full.colstart.resize(n1 + 1);
   for (int x = 0; x <= n1; ++x) full.colstart[x] = x * n2;
+  full.colpos.resize(nfull);
+  for (int s = 0; s < nfull; ++s) full.colpos[s] = s;
 
   qepw_plan* plan = new (std::nothrow) qepw_plan();
   if (!plan) return fail(QEPW_ERR_HIP, "qepw_plan_create: out of host memory");

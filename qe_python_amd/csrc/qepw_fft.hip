@@ -323,8 +323,16 @@ static void launch_split(bool real_mode, const Plan& p, const StickTables& st, c
 // is again position t + M s: only the kept slots are computed and stored.
 // The row pass chains backward x-FFT (DIF), V(r) multiply and forward x-FFT
 // (DIT) in registers -- the DIT consumes the DIF's output order -- at two
-// team exchanges per row.  Four workgroup barriers per plane.
+// team exchanges per row.
+//
+// LDS holds the plane and nothing else, so that more workgroups share a CU
+// (FP64 at the 72^3 workload: 42 KB, three per CU): a column FFT exchanges
+// through its own column (its values are in registers by then), and the row
+// pass reads a batch of rows into registers, after which the plane's storage
+// is free and each team exchanges through a line of it; the outputs wait in
+// registers until every team is done, then go back to their rows.
 constexpr int kTeamThreads = 256;
+constexpr int kPlaneTeamThreads = 256;
 
 template <int N> struct TeamGeom {
   static constexpr int L = team_L(N);
@@ -334,32 +342,44 @@ template <int N> struct TeamGeom {
   // input slots kept by the pruned kernel: the first two and the last two
   static constexpr unsigned all = (1u << L) - 1u;
   static constexpr unsigned pruned = L >= 6 ? (0x3u | (0x3u << (L - 2))) : all;
+  // the plane kernel's workgroup: kPlaneTeamThreads threads, plane_teams
+  // teams -- enough for the usual sphere's active columns in one round of
+  // each column pass and the rows in two rounds
+  static constexpr int plane_teams = (kPlaneTeamThreads / 64) * per_wave;
+  // rows a team holds in registers per batch of the plane kernel's row pass:
+  // all of its rows when there are at most three, and then the row pass
+  // exchanges through the plane's own storage; else one, with exchange lines
+  // of their own after the plane
+  static constexpr int rows = (N + plane_teams - 1) / plane_teams;
+  static constexpr bool rows_in_place = rows <= 3;
+  static constexpr int row_batch = rows_in_place ? rows : 1;
 };
 
 template <typename T, int N, bool PRUNE>
 __device__ __forceinline__ void planes_team_body(
     int z, int64_t b, cplx<T>* __restrict__ work, int64_t band_stride, int nst,
-    const int* __restrict__ stick_q, const int* __restrict__ colmap, int nxc,
+    const int* __restrict__ colpos, const int* __restrict__ colmap, int nxc,
     const T* __restrict__ V, const cplx<T>* __restrict__ tw) {
   using G = TeamGeom<N>;
-  constexpr int L = G::L, M = G::M;
+  constexpr int L = G::L, M = G::M, RB = G::row_batch;
   constexpr unsigned KEEP = PRUNE ? G::pruned : G::all;
   extern __shared__ unsigned char smem_raw[];
   // Q[c * NP + y]: active column c, padded stride NP = N + 1 so that the
-  // columns (and the teams' scratch lines) start on different LDS banks.
-  // Column nxc stays zero through the column passes: the row pass reads the
-  // inactive x positions from it and writes their (discarded) outputs to it.
+  // columns start on different LDS banks.  Column nxc stays zero through the
+  // column passes: the row pass reads the inactive x positions from it and
+  // writes their (discarded) outputs to it.
   constexpr int NP = N + 1;
   cplx<T>* Q = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* scratch_all = Q + (nxc + 1) * NP;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int team_in_wave = lane / M;
   const int t = lane - team_in_wave * M;
   const bool in_team = team_in_wave < G::per_wave;
   const int team = wave * G::per_wave + team_in_wave;
-  cplx<T>* S = scratch_all + team * NP;
   const int nq = (nxc + 1) * NP;
+  // the team's exchange line of the row pass: in the plane's storage when
+  // one batch holds every row, else after it
+  cplx<T>* E = Q + (G::rows_in_place ? 0 : nq) + team * NP;
   const cplx<T> zero = mk<T>(T(0), T(0));
   cplx<T>* w = work + b * band_stride + (int64_t)z * nst;
 
@@ -375,28 +395,32 @@ __device__ __forceinline__ void planes_team_body(
     cq[s] = (c < 0 ? nxc : c) * NP;
   }
 
-  for (int i = tid; i < nq; i += kTeamThreads) Q[i] = zero;
-  __syncthreads();
-  for (int s = tid; s < nst; s += kTeamThreads) {
-    const int q = stick_q[s];  // c * N + y
-    Q[q + q / N] = w[s];
-  }
-  __syncthreads();
+  // the zero column (ordered before the row pass by the barrier after the
+  // column pass, which does not touch it)
+  for (int i = tid; i < NP; i += kPlaneTeamThreads) Q[nxc * NP + i] = zero;
 
-  // backward y-FFT of every active column (in place in Q)
+  // backward y-FFT of every active column, its inputs read straight from the
+  // stick buffer (colpos: the stick at (c, y), or -1), in place in LDS: the
+  // column is the team's exchange line
   if (in_team) {
-    for (int c = team; c < nxc; c += G::teams) {
-      // a team's consecutive DIFs share its scratch line
-      if (c != team) wave_lds_sync();
+    for (int c = team; c < nxc; c += G::plane_teams) {
       cplx<T>* col = Q + c * NP;
+      const int* pos = colpos + c * N;
       cplx<T> bb[L], a[M];
       if (t < M) {
 #pragma unroll
-        for (int s = 0; s < L; ++s) bb[s] = ((KEEP >> s) & 1u) ? col[t + M * s] : zero;
+        for (int s = 0; s < L; ++s) {
+          bb[s] = zero;
+          if ((KEEP >> s) & 1u) {
+            const int st = pos[t + M * s];
+            if (st >= 0) bb[s] = w[st];
+          }
+        }
       }
-      dif_front<T, N, +1>(t, S, twr, bb);
+      dif_front<T, N, +1>(t, col, twr, bb);
       wave_lds_sync();
-      dif_back<T, N, +1>(t, S, a);
+      dif_back<T, N, +1>(t, col, a);
+      wave_lds_sync();
       if (t < L) {
 #pragma unroll
         for (int k = 0; k < M; ++k) col[t + L * k] = a[k];
@@ -405,73 +429,95 @@ __device__ __forceinline__ void planes_team_body(
   }
   __syncthreads();
 
-  // rows: backward x-FFT (inputs only at active columns), V, forward x-FFT
-  // (outputs only at active columns)
-  if (in_team) {
-    const T* vz = V + (int64_t)z * N * N;
-    for (int y = team; y < N; y += G::teams) {
-      // V(r) at this lane's backward outputs x = t + L k, fetched first
-      T vv[M];
+  // rows, RB per team and batch: backward x-FFT (inputs only at active
+  // columns), V, forward x-FFT (outputs only at active columns)
+  const T* vz = V + (int64_t)z * N * N;
+  for (int y0 = 0; y0 < N; y0 += RB * G::plane_teams) {
+    cplx<T> rows[RB][L];
 #pragma unroll
-      for (int k = 0; k < M; ++k) vv[k] = t < L ? vz[y * N + t + L * k] : T(0);
-      cplx<T> bb[L], a[M];
-      if (t < M) {
+    for (int r = 0; r < RB; ++r) {
+      const int y = y0 + team + r * G::plane_teams;
+      if (in_team && y < N && t < M) {
 #pragma unroll
-        for (int s = 0; s < L; ++s) bb[s] = ((KEEP >> s) & 1u) ? Q[cq[s] + y] : zero;
-      }
-      dif_front<T, N, +1>(t, S, twr, bb);
-      wave_lds_sync();
-      dif_back<T, N, +1>(t, S, a);
-      if (t < L) {
-#pragma unroll
-        for (int k = 0; k < M; ++k) a[k] = cscale(a[k], vv[k]);
-      }
-      dit_front<T, N, -1>(t, S, twr, a);
-      wave_lds_sync();
-      dit_back<T, N, -1>(t, S, bb);
-      if (t < M) {
-#pragma unroll
-        for (int s = 0; s < L; ++s)
-          if ((KEEP >> s) & 1u) Q[cq[s] + y] = bb[s];
+        for (int s = 0; s < L; ++s) rows[r][s] = ((KEEP >> s) & 1u) ? Q[cq[s] + y] : zero;
       }
     }
+    __syncthreads();
+    if (in_team) {
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int y = y0 + team + r * G::plane_teams;
+        if (y < N) {
+          // V(r) at this lane's backward outputs x = t + L k, fetched first
+          T vv[M];
+#pragma unroll
+          for (int k = 0; k < M; ++k) vv[k] = t < L ? vz[y * N + t + L * k] : T(0);
+          cplx<T> a[M];
+          // consecutive rows reuse the line: a DIF after a DIT writes only
+          // the slots the same lane read last
+          dif_front<T, N, +1>(t, E, twr, rows[r]);
+          wave_lds_sync();
+          dif_back<T, N, +1>(t, E, a);
+          if (t < L) {
+#pragma unroll
+            for (int k = 0; k < M; ++k) a[k] = cscale(a[k], vv[k]);
+          }
+          dit_front<T, N, -1>(t, E, twr, a);
+          wave_lds_sync();
+          dit_back<T, N, -1>(t, E, rows[r]);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int y = y0 + team + r * G::plane_teams;
+      if (in_team && y < N && t < M) {
+#pragma unroll
+        for (int s = 0; s < L; ++s)
+          if ((KEEP >> s) & 1u) Q[cq[s] + y] = rows[r][s];
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
 
-  // forward y-FFT of every active column; only the kept y slots, which hold
-  // every stick, are computed
+  // forward y-FFT of every active column, in place; only the kept y slots,
+  // which hold every stick, are computed, and the sticks among them go
+  // straight to the stick buffer
   if (in_team) {
-    for (int c = team; c < nxc; c += G::teams) {
-      if (c != team) wave_lds_sync();
+    for (int c = team; c < nxc; c += G::plane_teams) {
       cplx<T>* col = Q + c * NP;
+      const int* pos = colpos + c * N;
       cplx<T> a[M], bb[L];
       if (t < L) {
 #pragma unroll
         for (int s = 0; s < M; ++s) a[s] = col[t + L * s];
       }
-      dit_front<T, N, -1>(t, S, twr, a);
       wave_lds_sync();
-      dit_back<T, N, -1>(t, S, bb);
+      dit_front<T, N, -1>(t, col, twr, a);
+      wave_lds_sync();
+      dit_back<T, N, -1>(t, col, bb);
       if (t < M) {
 #pragma unroll
-        for (int k = 0; k < L; ++k)
-          if ((KEEP >> k) & 1u) col[t + M * k] = bb[k];
+        for (int k = 0; k < L; ++k) {
+          if ((KEEP >> k) & 1u) {
+            const int st = pos[t + M * k];
+            if (st >= 0) w[st] = bb[k];
+          }
+        }
       }
     }
   }
-  __syncthreads();
-  for (int s = tid; s < nst; s += kTeamThreads) {
-    const int q = stick_q[s];
-    w[s] = Q[q + q / N];
-  }
 }
 
+// at least three waves per SIMD: the FP64 plane's LDS admits three workgroups
+// per CU, and its registers are held to that
 template <typename T, int N, bool PRUNE>
-__global__ __launch_bounds__(kTeamThreads) void k_planes_team(
+__global__ __launch_bounds__(kPlaneTeamThreads, 3) void k_planes_team(
     cplx<T>* __restrict__ work, int64_t band_stride, int nst,
-    const int* __restrict__ stick_q, const int* __restrict__ colmap, int nxc,
+    const int* __restrict__ colpos, const int* __restrict__ colmap, int nxc,
     const T* __restrict__ V, const cplx<T>* __restrict__ tw) {
-  planes_team_body<T, N, PRUNE>(blockIdx.x, blockIdx.y, work, band_stride, nst, stick_q,
+  planes_team_body<T, N, PRUNE>(blockIdx.x, blockIdx.y, work, band_stride, nst, colpos,
                                 colmap, nxc, V, tw);
 }
 
@@ -636,7 +682,7 @@ template <typename T> struct PlaneGroupArgs {
   int nxc[kMaxFftGroup];
   cplx<T>* work[kMaxFftGroup];
   int64_t bstride[kMaxFftGroup];
-  const int* stickq[kMaxFftGroup];
+  const int* colpos[kMaxFftGroup];
   const int* colmap[kMaxFftGroup];
   const T* V[kMaxFftGroup];
   const cplx<T>* tw[kMaxFftGroup];
@@ -672,7 +718,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_sticks_group(const StickGroupA
 }
 
 template <typename T>
-__global__ __launch_bounds__(kTeamThreads) void k_planes_group(const PlaneGroupArgs<T> g) {
+__global__ __launch_bounds__(kPlaneTeamThreads, 3) void k_planes_group(const PlaneGroupArgs<T> g) {
   const int p = group_of(g.start, g.count, blockIdx.x);
   const int l = blockIdx.x - g.start[p];
   const int z = l % g.n3[p];
@@ -681,10 +727,10 @@ __global__ __launch_bounds__(kTeamThreads) void k_planes_group(const PlaneGroupA
 #define QEPW_PLANE_CASE(NN)                                                                     \
   case NN:                                                                                      \
     if (g.prune[p])                                                                             \
-      planes_team_body<T, NN, true>(z, b, g.work[p], g.bstride[p], g.nst[p], g.stickq[p],       \
+      planes_team_body<T, NN, true>(z, b, g.work[p], g.bstride[p], g.nst[p], g.colpos[p],       \
                                     g.colmap[p], g.nxc[p], g.V[p], g.tw[p]);                    \
     else                                                                                        \
-      planes_team_body<T, NN, false>(z, b, g.work[p], g.bstride[p], g.nst[p], g.stickq[p],      \
+      planes_team_body<T, NN, false>(z, b, g.work[p], g.bstride[p], g.nst[p], g.colpos[p],      \
                                      g.colmap[p], g.nxc[p], g.V[p], g.tw[p]);                   \
     break;
     QEPW_GROUP_SIZES(QEPW_PLANE_CASE)
@@ -717,6 +763,25 @@ static bool team_prunable(int n, unsigned mask) {
   return ok;
 }
 
+// LDS of the plane kernel (k_planes_team / k_planes_group): the active
+// columns and the zero column, holding an exchange line per team when the
+// row pass exchanges in place, else followed by them.
+template <typename T>
+static size_t plane_team_lds_bytes(int n, int nxc) {
+  size_t teams = 0;
+  bool in_place = false;
+#define QEPW_TEAMS_OF(NN)                  \
+  if (n == NN) {                           \
+    teams = TeamGeom<NN>::plane_teams;     \
+    in_place = TeamGeom<NN>::rows_in_place; \
+  }
+  QEPW_TEAM_SIZES(QEPW_TEAMS_OF)
+#undef QEPW_TEAMS_OF
+  if (teams == 0) return 0;
+  const size_t lines = in_place ? std::max((size_t)nxc + 1, teams) : (size_t)nxc + 1 + teams;
+  return lines * (size_t)(n + 1) * sizeof(cplx<T>);
+}
+
 template <typename T>
 static size_t team_lds_bytes(int n, int nxc) {
   size_t teams = 0;
@@ -735,16 +800,16 @@ template <typename T>
 static bool launch_planes_team(const Plan& p, const StickTables& st, cplx<T>* work,
                                int64_t bstride, const T* V, const cplx<T>* tw, int nb,
                                hipStream_t stream) {
-  if (p.n1 != p.n2 || !st.d_stickq || !st.d_colmap) return false;
-  const size_t lds = team_lds_bytes<T>(p.n1, st.nxcols);
+  if (p.n1 != p.n2 || !st.d_colpos || !st.d_colmap) return false;
+  const size_t lds = plane_team_lds_bytes<T>(p.n1, st.nxcols);
   if (lds == 0 || lds > (size_t)kMaxLdsBytes) return false;
   const dim3 grid(p.n3, nb);
 #define QEPW_LAUNCH_TEAM_K(NN, PR)                                                           \
   do {                                                                                       \
     hipFuncSetAttribute(reinterpret_cast<const void*>(&k_planes_team<T, NN, PR>),           \
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);               \
-    hipLaunchKernelGGL((k_planes_team<T, NN, PR>), grid, dim3(kTeamThreads), lds, stream,    \
-                       work, bstride, st.nst, st.d_stickq, st.d_colmap, st.nxcols, V, tw);  \
+    hipLaunchKernelGGL((k_planes_team<T, NN, PR>), grid, dim3(kPlaneTeamThreads), lds, stream, \
+                       work, bstride, st.nst, st.d_colpos, st.d_colmap, st.nxcols, V, tw);  \
   } while (0)
 #define QEPW_LAUNCH_TEAM(NN)                                                                 \
   if (p.n1 == NN) {                                                                          \
@@ -823,8 +888,8 @@ static void set_lds_attrs(size_t stick_bytes, size_t plane_bytes) {
 
 template <typename T>
 static bool team_available(const Plan& p, const StickTables& st) {
-  if (p.n1 != p.n2 || !st.d_stickq || !st.d_colmap) return false;
-  const size_t lds = team_lds_bytes<T>(p.n1, st.nxcols);
+  if (p.n1 != p.n2 || !st.d_colpos || !st.d_colmap) return false;
+  const size_t lds = plane_team_lds_bytes<T>(p.n1, st.nxcols);
   return lds > 0 && lds <= (size_t)kMaxLdsBytes;
 }
 
@@ -999,12 +1064,12 @@ int apply_local_group_t(int count, const Plan* const* plans, const void* const* 
       pa.nxc[j] = st.nxcols;
       pa.work[j] = sa.work[j];
       pa.bstride[j] = sa.bstride[j];
-      pa.stickq[j] = st.d_stickq;
+      pa.colpos[j] = st.d_colpos;
       pa.colmap[j] = st.d_colmap;
       pa.V[j] = static_cast<const T*>(potentials[i]);
       pa.tw[j] = twiddles<T>(p, 0);
       ptotal += (int64_t)p.n3 * nb[i];
-      plane_lds = std::max(plane_lds, team_lds_bytes<T>(p.n1, st.nxcols));
+      plane_lds = std::max(plane_lds, plane_team_lds_bytes<T>(p.n1, st.nxcols));
     }
     sa.start[m] = (int)stotal;
     pa.start[m] = (int)ptotal;
@@ -1017,7 +1082,7 @@ int apply_local_group_t(int count, const Plan* const* plans, const void* const* 
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_lds);
     hipLaunchKernelGGL((k_sticks_group<T, false>), dim3((unsigned)stotal), dim3(kTeamThreads),
                        stick_lds, stream, sa);
-    hipLaunchKernelGGL(k_planes_group<T>, dim3((unsigned)ptotal), dim3(kTeamThreads),
+    hipLaunchKernelGGL(k_planes_group<T>, dim3((unsigned)ptotal), dim3(kPlaneTeamThreads),
                        plane_lds, stream, pa);
     hipLaunchKernelGGL((k_sticks_group<T, true>), dim3((unsigned)stotal), dim3(kTeamThreads),
                        stick_lds, stream, sa);

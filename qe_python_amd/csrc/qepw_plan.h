@@ -32,6 +32,7 @@ struct StickTables {
   int* d_colmap = nullptr;  // n1: rank of x among active columns, or -1
   int* d_colstart = nullptr;  // nxcols+1: first stick of each active column (sticks
                               // are sorted by x, then y)
+  int* d_colpos = nullptr;  // nxcols * n2: stick at (column c, y), or -1
   unsigned slot_mask = 0;   // team-FFT input slots holding an active x or stick y
 };
 
