@@ -404,14 +404,17 @@ def batch_leg(args, device):
             npws.append(s.npw)
             roots.append(s.nroots)
     print(f"batch: {len(problems)} problems built", file=sys.stderr, flush=True)
+    from qe_python_amd.schedule import RoundStats
     times, outcomes = {}, {}
+    stats = RoundStats()
     for schedule in ("rounds", "threads"):
         davidson_batch(problems, workers=args.batch_workers, schedule=schedule,
                        device_output=True, **kwargs)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        outcomes[schedule] = davidson_batch(problems, workers=args.batch_workers,
-                                            schedule=schedule, device_output=True, **kwargs)
+        outcomes[schedule] = davidson_batch(
+            problems, workers=args.batch_workers, schedule=schedule, device_output=True,
+            stats=stats if schedule == "rounds" else None, **kwargs)
         torch.cuda.synchronize()
         times[schedule] = time.perf_counter() - t0
     t_batch = times["rounds"]
@@ -438,6 +441,17 @@ def batch_leg(args, device):
         "value": len(problems) / t_batch if valid else 0.0,
         "unit": "problems/s",
         "schedule": "rounds (lock-step, grouped FFT launches)",
+        # one round = every unfinished problem advanced to its next request;
+        # a grouped h_psi launches the 3 FFT-stage kernels for all its
+        # problems at once, where one by one each problem launches its own 3
+        "rounds": stats.rounds,
+        "h_psi_requests": stats.applies,
+        "grouped_h_psi_calls": stats.grouped_calls,
+        "fft_stage_launches": 3 * stats.grouped_calls,
+        "fft_stage_launches_one_by_one": 3 * stats.applies,
+        "problems_per_grouped_h_psi": stats.applies / max(1, stats.grouped_calls),
+        "host_waits": stats.waits,
+        "host_reads": stats.reads,
         "ms_batch": 1e3 * t_batch,
         "ms_threads": 1e3 * times["threads"],
         "ms_one_by_one": 1e3 * t_seq,

@@ -71,7 +71,8 @@ class BatchProblem:
 
 
 def davidson_batch(problems, *, workers: int | None = None, device=None,
-                   schedule: str = "rounds", **common) -> list[DavidsonResult]:
+                   schedule: str = "rounds", stats: "sched.RoundStats | None" = None,
+                   **common) -> list[DavidsonResult]:
     """Solve ``problems`` (a sequence of ``BatchProblem``) together.
 
     ``common`` holds ``davidson`` keywords shared by all problems (a
@@ -81,7 +82,8 @@ def davidson_batch(problems, *, workers: int | None = None, device=None,
     driving one problem at a time on its own stream).  Device tensors
     returned with ``device_output=True`` are complete when this returns.
     Problems must not share an operator object (each owns its device
-    workspace)."""
+    workspace).  ``stats`` (a ``schedule.RoundStats``) collects the rounds,
+    grouped H applications and reads of the "rounds" schedule."""
     problems = list(problems)
     if schedule not in ("rounds", "threads"):
         raise ValueError("schedule must be 'rounds' or 'threads'")
@@ -94,7 +96,7 @@ def davidson_batch(problems, *, workers: int | None = None, device=None,
     dev = dv.require_device(device)
     nat.load()
     if schedule == "rounds":
-        return _batch_rounds(problems, dev, common)
+        return _batch_rounds(problems, dev, common, stats)
     return _batch_threads(problems, dev, workers, common)
 
 
@@ -104,13 +106,13 @@ def _kwargs(problem, common) -> dict:
     return kwargs
 
 
-def _batch_rounds(problems, dev, common) -> list[DavidsonResult]:
+def _batch_rounds(problems, dev, common, stats) -> list[DavidsonResult]:
     with torch.cuda.device(dev):
         solves = [davidson_steps(p.operator, p.diagonal, p.number_of_roots,
                                  initial_vectors=p.initial_vectors,
                                  operator_into=p.operator_into, **_kwargs(p, common))
                   for p in problems]
-        results = sched.run_many(solves, dev)
+        results = sched.run_many(solves, dev, stats)
         torch.cuda.current_stream(dev).synchronize()
     return results
 

@@ -83,7 +83,23 @@ def run(steps, device: torch.device):
             return stop.value
 
 
-def run_many(solves, device: torch.device) -> list:
+class RoundStats:
+    """What ``run_many`` did: rounds, H applications and how they were
+    grouped, device-to-host reads (each round's reads share one wait)."""
+
+    def __init__(self):
+        self.rounds = 0
+        self.applies = 0            # Apply requests
+        self.grouped_calls = 0      # device_apply_group calls (one per precision per round)
+        self.reads = 0
+        self.waits = 0
+
+    def as_dict(self) -> dict:
+        return {"rounds": self.rounds, "applies": self.applies,
+                "grouped_calls": self.grouped_calls, "reads": self.reads, "waits": self.waits}
+
+
+def run_many(solves, device: torch.device, stats: RoundStats | None = None) -> list:
     """Drive independent solves in rounds; returns their results in order.
 
     Each round: every unfinished solve runs until its next request; the
@@ -113,6 +129,8 @@ def run_many(solves, device: torch.device) -> list:
                 if all(isinstance(r.op, PlaneWaveHamiltonian) for r in group):
                     PlaneWaveHamiltonian.device_apply_group(
                         [(r.op, r.src, r.dst) for r in group])
+                    if stats is not None:
+                        stats.grouped_calls += 1
                 else:
                     for r in group:
                         r.op.device_apply(r.src, r.dst)
@@ -128,6 +146,11 @@ def run_many(solves, device: torch.device) -> list:
                 stream.synchronize()
                 for i, host in staged:
                     values[i] = host.numpy()
+            if stats is not None:
+                stats.rounds += 1
+                stats.applies += len(applies)
+                stats.reads += len(staged)
+                stats.waits += 1 if staged else 0
             for i in list(pending):
                 try:
                     pending[i] = solves[i].send(values[i])

@@ -60,8 +60,20 @@ def sequential(systems):
 @pytest.mark.parametrize("schedule", ["rounds", "threads"])
 def test_batch_matches_oracle_and_sequential(systems, sequential, schedule):
     from qe_python_amd.batch import davidson_batch
+    from qe_python_amd.schedule import RoundStats
     golden = np.load(GOLDEN)
-    results = davidson_batch(batch_problems(systems), workers=4, schedule=schedule, **KWARGS)
+    stats = RoundStats()
+    results = davidson_batch(batch_problems(systems), workers=4, schedule=schedule,
+                             stats=stats, **KWARGS)
+    if schedule == "rounds":
+        # every round groups its H applications by precision: at most two
+        # grouped calls per round, and far fewer calls than requests
+        print(stats.as_dict())
+        assert stats.rounds >= max(r.iterations for r in results)
+        assert stats.grouped_calls <= 2 * stats.rounds
+        assert stats.applies >= 2 * stats.grouped_calls
+    else:
+        assert stats.rounds == 0
     for (name, s, h), res in zip(systems, results):
         assert int(golden[f"{name}_npw"]) == s.npw
         err = np.abs(res.eigenvalues - golden[name])
