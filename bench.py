@@ -82,7 +82,8 @@ def parse():
 
 def fft_probe(calls):
     """Calibration copy (k_permute, identity: reads and writes COPY_BYTES each)
-    followed by ``calls`` FFT-phase launches over 128 bands."""
+    followed by ``calls`` FFT-phase launches over 128 bands in FP64, then
+    ``calls`` in FP32 (complex64, the mixed form's first phase)."""
     import torch
 
     from qe_python_amd import device as dv
@@ -97,14 +98,20 @@ def fft_probe(calls):
     src = torch.zeros((16, rows), dtype=torch.complex128, device=device)
     dst = torch.empty_like(src)
     dv.permute_rows(0, None, src, dst, rows)  # COPY_BYTES = 16 * 2^20 * 16 = 256 MiB
-    x = ham.to_device(system.trial[:, :128])
-    y = torch.empty_like(x)
-    res = ham._resources(x.dtype)
-    for _ in range(calls):
-        ham.plan.apply_local(x, y, res.potential, res.kinetic, ham.workspace)
+    n1, n2, n3 = ham.plan.shape
+    algorithmic = {}
+    for dtype, name in ((torch.complex128, "fp64"), (torch.complex64, "fp32")):
+        x = ham.to_device(system.trial[:, :128], dtype)
+        y = torch.empty_like(x)
+        res = ham._resources(x.dtype)
+        for _ in range(calls):
+            ham.plan.apply_local(x, y, res.potential, res.kinetic, ham.workspace)
+        algorithmic[name] = fft_phase_bytes(128, ham.npw, ham.plan.sticks, n3, n1 * n2 * n3,
+                                            16 if name == "fp64" else 8)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": 16 * rows * 16, "calls": calls, "nb": 128,
-                      "npw": ham.npw, "sticks": ham.plan.sticks}))
+                      "npw": ham.npw, "sticks": ham.plan.sticks,
+                      "algorithmic_bytes": algorithmic}))
     return 0
 
 
@@ -757,18 +764,19 @@ def main():
     total_ms = sum(a.elapsed_time(b) for _, _, a, b in head_events)
     calls = len(head_events)
     achieved = total_bytes / (total_ms * 1e-3) if total_ms > 0 else 0.0
-    # HBM bytes per launch: the traffic/algorithmic ratio of the committed
-    # rocprofv3 FETCH_SIZE/WRITE_SIZE passes (128-band FP64 calls) applied to
-    # this run's average algorithmic bytes per call
+    # HBM bytes per launch: the traffic/algorithmic ratio per precision of
+    # the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes (128-band FP64 and
+    # FP32 calls), applied to each call of this run by its precision
     traffic = None
     pmc = ROOT / "profiles" / "fft_phase_medium_traffic.json"
-    if pmc.exists():
-        probe = json.loads(pmc.read_text())
-        ratio = probe.get("traffic_over_algorithmic")
-        if ratio is None and probe.get("algorithmic_bytes_per_call"):
-            ratio = probe["traffic_bytes_per_call"] / probe["algorithmic_bytes_per_call"]
-        if ratio is not None and calls:
-            traffic = ratio * total_bytes / calls
+    if pmc.exists() and calls:
+        ratios = {name: rec["traffic_over_algorithmic"]
+                  for name, rec in json.loads(pmc.read_text()).get("precisions", {}).items()}
+        if {"fp64", "fp32"} <= set(ratios):
+            traffic = sum(ratios["fp64" if dt == torch.complex128 else "fp32"] *
+                          fft_phase_bytes(nb, plan.npw, plan.sticks, n3, n1 * n2 * n3,
+                                          16 if dt == torch.complex128 else 8)
+                          for nb, dt, _, _ in head_events) / calls
 
     line = {
         "metric": "Davidson solves/s to 1e-8 Ry (mixed vs FP64 GPU vs CPU); h_psi HBM GB/s",
