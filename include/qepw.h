@@ -438,6 +438,78 @@ int qepw_sign_iteration(void* x, void* x2, void* y, int64_t ld, int n, const dou
                         int nalpha, int* in_x2, void* workspace, size_t workspace_bytes,
                         void* stream);
 
+/* ---- native Rayleigh-Ritz steps ------------------------------------------
+ * The reduced eigensolve of one Davidson iteration (replacing, with the device
+ * spectral division above, the reference's zhegvd of the copied pencil:
+ * generalized_eigh, qepy_pw/_native_fft.pyx:667-691, called from
+ * qepy_pw/diagonalization.py:699-726) as two host calls that launch their
+ * kernels on ``stream``, read back the few small matrices they need (pinned
+ * memory, waiting for the stream), factorise them with the host LAPACK and
+ * upload the factors.  All matrices are column-major complex128 with leading
+ * dimension ``maxsub``.
+ *
+ * qepw_set_host_lapack: the LAPACK routines used (Fortran calling convention:
+ *   zpotrf, ztrtri, zheevd, cheevd), e.g. those of the caller's NumPy/SciPy;
+ *   required before the calls below (QEPW_ERR_UNSUPPORTED otherwise).
+ * qepw_rr_workspace: device workspace bytes of both calls.
+ * qepw_rr_standardize: with S = L L^H and W = L^-1 known for rows and columns
+ *   [0, first), extend W to [0, active) and write rows [first, active) of
+ *   A = W H W^H (A Hermitian-completed).  *not_pd = 1 (and nothing written)
+ *   when the new Schur complement of S is not positive definite.
+ * qepw_rr_spectral: the lowest k eigenpairs of the n x n A from the shift
+ *   ``sigma`` with ``gap`` the spacing around it: the scaled Newton-Schulz
+ *   iteration to sign(A - sigma I) (scratch X, X2, Y), a range sample of the
+ *   projector below the shift from the previous eigenvectors ``u_prev``
+ *   (k columns, when have_prev) and Gaussian columns ``gauss`` (maxsub
+ *   columns), a Rayleigh-Ritz on it.  ``rb``: a device buffer of
+ *   2 maxsub + 4 + 2 maxsub^2 doubles (sign check, bound, Gram matrix).  On
+ *   status 0: values[0:m_below] (host) the eigenvalues below the shift,
+ *   ``coef`` (n x k, leading dimension n) = W^H U for the eigenvectors U of
+ *   A, and U stored in u_prev.  status 1: the sign iteration did not
+ *   converge; 2: fewer than k eigenvalues below the shift; 3: the sample's
+ *   Gram matrix is not positive definite; 4: the host eigensolve failed.
+ *   ``bound_prev`` > 0 estimates ||A - sigma I||_1 as twice that value (else
+ *   it is read back); ``bound`` returns the exact value of this call.
+ */
+typedef struct qepw_rr_spectral_args {
+  int n;
+  int k;
+  int maxsub;
+  int count_hint;  /* range sample columns to try first */
+  int host_single; /* the host eigensolve in complex64 */
+  int have_prev;
+  double sigma;
+  double gap;
+  double bound_prev;
+  void* A;
+  void* W;
+  void* X;
+  void* X2;
+  void* Y;
+  void* u_prev;
+  void* gauss;
+  void* coef;
+  double* rb;
+  void* work;
+  size_t work_bytes;
+  void* stream;
+} qepw_rr_spectral_args;
+
+typedef struct qepw_rr_spectral_result {
+  int status;
+  int m_below;
+  int ns_steps;
+  int slow_shift;
+  double bound;
+} qepw_rr_spectral_result;
+
+int qepw_set_host_lapack(void* zpotrf, void* ztrtri, void* zheevd, void* cheevd);
+size_t qepw_rr_workspace(int maxsub);
+int qepw_rr_standardize(void* h, void* s, void* w, void* a, int maxsub, int first, int active,
+                        int* not_pd, void* workspace, size_t workspace_bytes, void* stream);
+int qepw_rr_spectral(const qepw_rr_spectral_args* args, double* values,
+                     qepw_rr_spectral_result* result);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,7 +110,48 @@ SIGNATURES = [
     ("qepw_sign_iteration", c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int,
                                     POINTER(c_double), c_int, POINTER(c_int), c_void_p,
                                     c_size_t, c_void_p]),
+    ("qepw_set_host_lapack", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("qepw_rr_workspace", c_size_t, [c_int]),
+    ("qepw_rr_standardize", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                                    POINTER(c_int), c_void_p, c_size_t, c_void_p]),
+    ("qepw_rr_spectral", c_int, [c_void_p, c_void_p, c_void_p]),
 ]
+
+
+class RRSpectralArgs(ctypes.Structure):
+    """``qepw_rr_spectral_args`` of include/qepw.h."""
+
+    _fields_ = [("n", c_int), ("k", c_int), ("maxsub", c_int), ("count_hint", c_int),
+                ("host_single", c_int), ("have_prev", c_int), ("sigma", c_double),
+                ("gap", c_double), ("bound_prev", c_double), ("A", c_void_p), ("W", c_void_p),
+                ("X", c_void_p), ("X2", c_void_p), ("Y", c_void_p), ("u_prev", c_void_p),
+                ("gauss", c_void_p), ("coef", c_void_p), ("rb", c_void_p), ("work", c_void_p),
+                ("work_bytes", c_size_t), ("stream", c_void_p)]
+
+
+class RRSpectralResult(ctypes.Structure):
+    """``qepw_rr_spectral_result`` of include/qepw.h."""
+
+    _fields_ = [("status", c_int), ("m_below", c_int), ("ns_steps", c_int),
+                ("slow_shift", c_int), ("bound", c_double)]
+
+
+def _register_host_lapack(lib) -> None:
+    """Hand the library the LAPACK routines of SciPy's own LAPACK (the
+    function pointers ``scipy.linalg.cython_lapack`` exports), used by the
+    native Rayleigh-Ritz steps for their small host factorisations."""
+    import scipy.linalg.cython_lapack as cl
+    get_name = ctypes.pythonapi.PyCapsule_GetName
+    get_name.restype = ctypes.c_char_p
+    get_name.argtypes = [ctypes.py_object]
+    get_ptr = ctypes.pythonapi.PyCapsule_GetPointer
+    get_ptr.restype = c_void_p
+    get_ptr.argtypes = [ctypes.py_object, ctypes.c_char_p]
+    ptrs = []
+    for name in ("zpotrf", "ztrtri", "zheevd", "cheevd"):
+        capsule = cl.__pyx_capi__[name]
+        ptrs.append(get_ptr(capsule, get_name(capsule)))
+    check(lib.qepw_set_host_lapack(*ptrs), "qepw_set_host_lapack")
 
 _LIB: ctypes.CDLL | None = None
 
@@ -134,6 +175,7 @@ def load() -> ctypes.CDLL:
         fn.restype = restype
         fn.argtypes = argtypes
     _LIB = lib
+    _register_host_lapack(lib)
     # the committed GEMM tile table (qe_python_amd.autotune); QEPW_GEMM_TABLE
     # names another table, or "none" for the built-in choice
     table = os.environ.get("QEPW_GEMM_TABLE")

@@ -763,6 +763,47 @@ int qepw_sign_iteration(void* x, void* x2, void* y, int64_t ld, int n, const dou
                 "qepw_sign_iteration");
 }
 
+int qepw_set_host_lapack(void* zpotrf, void* ztrtri, void* zheevd, void* cheevd) {
+  if (!zpotrf || !ztrtri || !zheevd || !cheevd)
+    return fail(QEPW_ERR_ARG, "qepw_set_host_lapack: NULL routine");
+  qepw::set_host_lapack(zpotrf, ztrtri, zheevd, cheevd);
+  return QEPW_OK;
+}
+
+size_t qepw_rr_workspace(int maxsub) { return maxsub > 0 ? qepw::rr_workspace_bytes(maxsub) : 0; }
+
+int qepw_rr_standardize(void* h, void* s, void* w, void* a, int maxsub, int first, int active,
+                        int* not_pd, void* workspace, size_t workspace_bytes, void* stream) {
+  if (!h || !s || !w || !a || !not_pd || maxsub < 1 || first < 0 || first > active ||
+      active > maxsub || (active > first && !workspace))
+    return fail(QEPW_ERR_ARG, "qepw_rr_standardize: bad argument");
+  if (!qepw::host_lapack_ready())
+    return fail(QEPW_ERR_UNSUPPORTED,
+                "qepw_rr_standardize: no host LAPACK registered (qepw_set_host_lapack)");
+  const int rc = qepw::rr_standardize(h, s, w, a, maxsub, first, active, not_pd, workspace,
+                                      workspace_bytes, static_cast<hipStream_t>(stream));
+  if (rc != QEPW_OK) g_error = "qepw_rr_standardize: " + qepw::rr_message();
+  return rc;
+}
+
+int qepw_rr_spectral(const qepw_rr_spectral_args* args, double* values,
+                     qepw_rr_spectral_result* result) {
+  if (!args || !values || !result) return fail(QEPW_ERR_ARG, "qepw_rr_spectral: NULL argument");
+  const qepw_rr_spectral_args& g = *args;
+  if (g.k < 1 || g.n < g.k || g.n > g.maxsub || g.count_hint < 1 || !g.A || !g.W || !g.X ||
+      !g.X2 || !g.Y || !g.u_prev || !g.gauss || !g.coef || !g.rb || !g.work ||
+      g.X == g.X2 || g.X == g.Y || g.X2 == g.Y || !(g.gap > 0.0))
+    return fail(QEPW_ERR_ARG, "qepw_rr_spectral: bad argument");
+  if (g.work_bytes < qepw::rr_workspace_bytes(g.maxsub))
+    return fail(QEPW_ERR_WORKSPACE, "qepw_rr_spectral: workspace too small");
+  if (!qepw::host_lapack_ready())
+    return fail(QEPW_ERR_UNSUPPORTED,
+                "qepw_rr_spectral: no host LAPACK registered (qepw_set_host_lapack)");
+  const int rc = qepw::rr_spectral(args, values, result);
+  if (rc != QEPW_OK) g_error = "qepw_rr_spectral: " + qepw::rr_message();
+  return rc;
+}
+
 int qepw_set_diagonal(void* a, int64_t lda, int n, const double* d, void* stream) {
   if (n < 0 || (n > 0 && (!a || lda < n)))
     return fail(QEPW_ERR_ARG, "qepw_set_diagonal: bad argument");

@@ -1,6 +1,8 @@
 // Internal declarations shared by the HIP translation units.
 #pragma once
 
+#include <string>
+
 #include "qepw_common.h"
 #include "qepw_plan.h"
 #include "../../include/qepw.h"
@@ -106,5 +108,14 @@ int set_diagonal(void* a, int64_t lda, int n, const double* d, hipStream_t strea
 size_t sign_iteration_workspace(int n);
 int sign_iteration(void* x, void* x2, void* y, int64_t ld, int n, const double* alphas,
                    int nalpha, int* in_x2, void* ws, size_t ws_bytes, hipStream_t stream);
+
+// qepw_rr.hip
+void set_host_lapack(void* zpotrf, void* ztrtri, void* zheevd, void* cheevd);
+bool host_lapack_ready();
+const std::string& rr_message();
+size_t rr_workspace_bytes(int maxsub);
+int rr_standardize(void* H, void* S, void* W, void* A, int maxsub, int first, int active,
+                   int* not_pd, void* work, size_t work_bytes, hipStream_t stream);
+int rr_spectral(const qepw_rr_spectral_args* a, double* values, qepw_rr_spectral_result* out);
 
 }  // namespace qepw

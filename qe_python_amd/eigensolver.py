@@ -48,7 +48,7 @@ from threadpoolctl import ThreadpoolController
 
 from . import _native as nat
 from . import device as dv
-from .schedule import Read, blocking
+from .schedule import Read, blocking, may_block
 
 C128 = torch.complex128
 _THREADPOOLS = None
@@ -92,6 +92,12 @@ def _scaled_ns_alphas(lower: float, tol: float = 1e-15, limit: int = 60) -> list
 
 
 _GAUSS: dict = {}
+
+# Run the standardisation and the spectral solve as native calls when the
+# driver waits on reads itself (qepw_rr_standardize / qepw_rr_spectral: the
+# same steps, their host LAPACK calls inside the library); False keeps the
+# Python steps everywhere
+NATIVE_RR = True
 
 
 def _gaussian_columns(size: int, device) -> torch.Tensor:
@@ -273,13 +279,20 @@ class DeviceRayleighRitz:
             self.stats["host"] += 1
             return values, coef
         k = self.nroots
-        # the host factorisations run on one BLAS thread each (a limit that
-        # is process-wide: it is never held across a yield, where another
-        # solve may run)
-        yield from self._standardize(active)
+        # under a driver that waits on each read itself, the standardisation
+        # and the spectral solve run as one native call each (their host
+        # round trips inside the library); under run_many they yield
+        native = NATIVE_RR and may_block()
+        if native:
+            self._standardize_native(active)
+        else:
+            yield from self._standardize(active)
         out = None
         if self.theta is not None and active > k + 8:
-            out = yield from self._spectral(active)
+            if native:
+                out = self._spectral_native(active)
+            else:
+                out = yield from self._spectral(active)
         else:
             self.stats["skipped"] += 1
         if out is not None:
@@ -359,10 +372,11 @@ class DeviceRayleighRitz:
         dv.set_block(self.u_prev, self.maxsub, k)
         dv.copy_columns(U, self.u_prev, n, k)
 
-    def _spectral(self, n):
+    def _choose_shift(self):
+        """(sigma, lo, hi): the shift of the next spectral solve and the
+        eigenvalue bracket around it, from the previous Ritz values; None
+        when too few are known."""
         k = self.nroots
-        st = dv.stream_handle(self.device)
-        lib = nat.load()
         theta = self.theta
         if self._can_keep_shift():
             # keep the previous shift: the subspace only grew since, so by
@@ -408,6 +422,77 @@ class DeviceRayleighRitz:
                 gap = theta[k] - theta[k - 1]
                 lo_v, hi_v = theta[k - 1], theta[k - 1] + 0.1 * gap
                 sigma = 0.5 * (lo_v + hi_v)
+        return sigma, lo_v, hi_v
+
+    def _standardize_native(self, active: int):
+        """``_standardize`` as one native call (qepw_rr_standardize)."""
+        first = self.standard
+        if first >= active:
+            return
+        lib = nat.load()
+        buf = self.ws.get(int(lib.qepw_rr_workspace(self.maxsub)))
+        not_pd = ctypes.c_int(0)
+        nat.check(lib.qepw_rr_standardize(self.H.data_ptr(), self.S.data_ptr(),
+                                          self.W.data_ptr(), self.A.data_ptr(), self.maxsub,
+                                          first, active, ctypes.byref(not_pd), buf.data_ptr(),
+                                          buf.numel(), dv.stream_handle(self.device)),
+                  "qepw_rr_standardize")
+        if not_pd.value:
+            raise np.linalg.LinAlgError("zhegvd failed: overlap not positive definite")
+        self.standard = active
+
+    def _spectral_native(self, n):
+        """``_spectral`` with the sign iteration, the range sample and the
+        Rayleigh-Ritz on it as one native call (qepw_rr_spectral)."""
+        k = self.nroots
+        shift = self._choose_shift()
+        if shift is None:
+            return None
+        sigma, lo_v, hi_v = shift
+        lib = nat.load()
+        have_prev = self.u_prev is not None
+        if not have_prev:
+            self.u_prev = torch.empty((k, self.maxsub), dtype=C128, device=self.device)
+        coef = torch.empty((k, n), dtype=C128, device=self.device)
+        buf = self.ws.get(int(lib.qepw_rr_workspace(self.maxsub)))
+        args = nat.RRSpectralArgs(
+            n=n, k=k, maxsub=self.maxsub, count_hint=len(self.theta) + 24,
+            host_single=int(self.host_single), have_prev=int(have_prev), sigma=sigma,
+            gap=hi_v - lo_v, bound_prev=self._bound_prev or 0.0, A=self.A.data_ptr(),
+            W=self.W.data_ptr(), X=self.X.data_ptr(), X2=self.X2.data_ptr(),
+            Y=self.Y.data_ptr(), u_prev=self.u_prev.data_ptr(),
+            gauss=_gaussian_columns(self.maxsub, self.device).data_ptr(),
+            coef=coef.data_ptr(), rb=self.rb.data_ptr(), work=buf.data_ptr(),
+            work_bytes=buf.numel(), stream=dv.stream_handle(self.device))
+        values = np.empty(self.maxsub, dtype=np.float64)
+        res = nat.RRSpectralResult()
+        nat.check(lib.qepw_rr_spectral(ctypes.byref(args), values.ctypes.data,
+                                       ctypes.byref(res)), "qepw_rr_spectral")
+        self.stats["ns_iterations"] += res.ns_steps
+        self._bound_prev = res.bound
+        self._slow_shift = bool(res.slow_shift)
+        if res.status == 4:
+            raise np.linalg.LinAlgError("heevd failed on the reduced Rayleigh-Ritz matrix")
+        if res.status:
+            self.stats[{1: "ns_unconverged", 2: "too_few_below", 3: "gram_not_pd"}[res.status]] += 1
+            if not have_prev:
+                self.u_prev = None
+            return None
+        w = values[:res.m_below].copy()
+        self._note_drift(w)
+        self.theta = w
+        self.sigma = sigma
+        return w[:k].copy(), coef
+
+    def _spectral(self, n):
+        k = self.nroots
+        st = dv.stream_handle(self.device)
+        lib = nat.load()
+        theta = self.theta
+        shift = self._choose_shift()
+        if shift is None:
+            return None
+        sigma, lo_v, hi_v = shift
         A = self.block(self.A, 0, n, 0, n)
         X, X2, Y = (self.block(m, 0, n, 0, n) for m in (self.X, self.X2, self.Y))
         nat.check(lib.qepw_shift_scale(A.data_ptr(), self.maxsub, n, sigma, X.data_ptr(),

@@ -21,6 +21,9 @@ another problem's kernels.
 
 from __future__ import annotations
 
+import threading
+from contextlib import contextmanager
+
 import torch
 
 from . import device as dv
@@ -46,17 +49,48 @@ class Apply:
         self.dst = dst
 
 
+_TLS = threading.local()
+
+
+def may_block() -> bool:
+    """Whether the driver of the running steps waits on each read itself
+    (``run``, ``blocking``): steps may then do their host round trips inside
+    one native call instead of yielding them (``run_many`` needs the yields
+    to interleave its solves)."""
+    return getattr(_TLS, "blocking", 0) > 0
+
+
+@contextmanager
+def _blocking_driver():
+    _TLS.blocking = getattr(_TLS, "blocking", 0) + 1
+    try:
+        yield
+    finally:
+        _TLS.blocking -= 1
+
+
+@contextmanager
+def _yielding_driver():
+    prev = getattr(_TLS, "blocking", 0)
+    _TLS.blocking = 0
+    try:
+        yield
+    finally:
+        _TLS.blocking = prev
+
+
 def blocking(steps):
     """Drive a generator that only reads (no H applications), with blocking
     copies, on whatever stream is current."""
-    try:
-        request = next(steps)
-        while True:
-            if type(request) is not Read:
-                raise TypeError("blocking() drives read-only steps")
-            request = steps.send(request.tensor.cpu().numpy())
-    except StopIteration as stop:
-        return stop.value
+    with _blocking_driver():
+        try:
+            request = next(steps)
+            while True:
+                if type(request) is not Read:
+                    raise TypeError("blocking() drives read-only steps")
+                request = steps.send(request.tensor.cpu().numpy())
+        except StopIteration as stop:
+            return stop.value
 
 
 def _one_blas_thread():
@@ -69,7 +103,7 @@ def run(steps, device: torch.device):
     host LAPACK work of the solve (small factorisations between device
     steps) runs on one BLAS thread: the limit is process-wide, so the driver
     holds it for the whole drive rather than the steps around each call."""
-    with dv.fixed_stream(device), _one_blas_thread():
+    with dv.fixed_stream(device), _one_blas_thread(), _blocking_driver():
         try:
             request = next(steps)
             while True:
@@ -112,7 +146,7 @@ def run_many(solves, device: torch.device, stats: RoundStats | None = None) -> l
 
     results = [None] * len(solves)
     pending = {}                    # index -> request
-    with dv.fixed_stream(device), _one_blas_thread():
+    with dv.fixed_stream(device), _one_blas_thread(), _yielding_driver():
         stream = torch.cuda.current_stream(device)
         for i, steps in enumerate(solves):
             try:

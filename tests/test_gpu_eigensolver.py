@@ -21,10 +21,15 @@ def dev_block(a):
     return torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).to(DEV)
 
 
+@pytest.mark.parametrize("native", [True, False], ids=["native", "python"])
 @pytest.mark.parametrize("nroots,blocks", [(24, [40, 24, 24, 24]), (64, [80, 64, 64, 64])])
-def test_spectral_rayleigh_ritz_matches_host(nroots, blocks):
+def test_spectral_rayleigh_ritz_matches_host(nroots, blocks, native, monkeypatch):
+    """Both forms of the standardisation and spectral solve: the native calls
+    (qepw_rr_standardize / qepw_rr_spectral) and the Python steps."""
     from qe_python_amd import device as dv
+    from qe_python_amd import eigensolver
     from qe_python_amd.eigensolver import DeviceRayleighRitz
+    monkeypatch.setattr(eigensolver, "NATIVE_RR", native)
     rng = np.random.default_rng(nroots)
     npw = 900
     q, _ = np.linalg.qr(rng.standard_normal((npw, npw)) + 1j * rng.standard_normal((npw, npw)))
@@ -113,3 +118,24 @@ def test_sign_iteration_entry_point(n, ld):
                                           bufs[2].data_ptr(), ld, n, coeffs, 1,
                                           ctypes.byref(which), ws.data_ptr(), ws.numel(), st),
                   "sign")
+
+
+@pytest.mark.parametrize("native", [True, False], ids=["native", "python"])
+def test_dependent_basis_raises_like_zhegvd(native, monkeypatch):
+    """A basis with a repeated column has a singular overlap: the
+    standardisation reports it as the reference's zhegvd failure."""
+    from qe_python_amd import device as dv
+    from qe_python_amd import eigensolver
+    from qe_python_amd.eigensolver import DeviceRayleighRitz
+    monkeypatch.setattr(eigensolver, "NATIVE_RR", native)
+    rng = np.random.default_rng(5)
+    npw, nb = 200, 12
+    blk = rng.standard_normal((npw, nb)) + 1j * rng.standard_normal((npw, nb))
+    blk[:, 7] = blk[:, 3]
+    V = dev_block(blk)
+    HV = dev_block(blk * np.linspace(0.0, 1.0, nb)[None, :])
+    rr = DeviceRayleighRitz(nb, 4, torch.device(DEV), dv.Workspace(torch.device(DEV)))
+    rr.update(V, HV, 0, nb, npw)
+    with pytest.raises(np.linalg.LinAlgError, match="zhegvd"):
+        rr.solve(nb)
+
