@@ -8,19 +8,23 @@ launches are too small to fill 256 CUs late in a solve when few roots remain
 unconverged.  ``davidson_batch`` runs the problems together, in one of two
 schedules:
 
-* ``"rounds"`` (default): every problem's Davidson is the generator
-  ``davidson_steps``; ``schedule.run_many`` advances all of them in rounds
-  on one host thread and one stream.  In a round, H is applied to every
-  requested block of one precision at once -- the FFT phase of all problems
-  as one grouped launch per stage (``qepw_apply_local_group``: sticks,
-  planes, sticks, each workgroup picking its problem from a prefix table) --
-  and every requested host value (residual norms, reduced matrices) is read
-  back after one wait, so one problem's host factorisations run while the
-  device executes the others' kernels.
-* ``"threads"``: every problem's unchanged ``davidson`` on its own HIP
-  stream (default priority) from its own host thread; kernels of different
-  problems overlap through the hardware queues, each thread's host LAPACK
-  work runs on its own core with a single-threaded BLAS.
+* ``"threads"`` (default): every problem's unchanged ``davidson`` on its own
+  HIP stream (default priority) from one of ``workers`` host threads;
+  kernels of different problems overlap through the hardware queues, and
+  the host side of each solve -- the reduced eigensolve's native calls
+  (``qepw_rr_standardize`` / ``qepw_rr_spectral``, which release the GIL),
+  its LAPACK on a single-threaded BLAS -- runs while the other threads
+  launch.  Two workers measured best on one MI355X with the 24-problem
+  uneven workload (1.5x over one by one; more contend for the GIL).
+* ``"rounds"``: every problem's Davidson is the generator
+  ``davidson_steps``; ``schedule.run_many`` advances all of them in lock-step
+  rounds on one host thread and one stream.  In a round, H is applied to
+  every requested block of one precision at once -- the FFT phase of all
+  problems as one grouped launch per stage (``qepw_apply_local_group``:
+  sticks, planes, sticks, each workgroup picking its problem from a prefix
+  table) -- and every requested host value (residual norms, reduced
+  matrices) is read back after one wait.  Its reduced eigensolves stay in
+  the yielding Python steps, so its host side is single-threaded.
 
 Results are identical to solving the problems one by one in both schedules:
 every kernel is deterministic (fixed-order reductions, no atomics), the
@@ -71,15 +75,15 @@ class BatchProblem:
 
 
 def davidson_batch(problems, *, workers: int | None = None, device=None,
-                   schedule: str = "rounds", stats: "sched.RoundStats | None" = None,
+                   schedule: str = "threads", stats: "sched.RoundStats | None" = None,
                    **common) -> list[DavidsonResult]:
     """Solve ``problems`` (a sequence of ``BatchProblem``) together.
 
     ``common`` holds ``davidson`` keywords shared by all problems (a
-    problem's ``options`` override them).  ``schedule`` is "rounds" (all
-    problems in lock-step rounds with grouped H applications, on the
-    current stream) or "threads" (``workers`` host threads, default 2, each
-    driving one problem at a time on its own stream).  Device tensors
+    problem's ``options`` override them).  ``schedule`` is "threads"
+    (``workers`` host threads, default 2, each driving one problem at a time
+    on its own stream) or "rounds" (all problems in lock-step rounds with
+    grouped H applications, on the current stream).  Device tensors
     returned with ``device_output=True`` are complete when this returns.
     Problems must not share an operator object (each owns its device
     workspace).  ``stats`` (a ``schedule.RoundStats``) collects the rounds,
