@@ -509,6 +509,18 @@ int qepw_rr_standardize(void* h, void* s, void* w, void* a, int maxsub, int firs
                         int* not_pd, void* workspace, size_t workspace_bytes, void* stream);
 int qepw_rr_spectral(const qepw_rr_spectral_args* args, double* values,
                      qepw_rr_spectral_result* result);
+/* qepw_cholesky_qr: orthonormalise the nb columns of an npw x nb block
+ * (complex128 or complex64, leading dimension ldb) by Cholesky QR -- Gram
+ * matrix on the device, its Cholesky factor in FP64 on the host, out =
+ * block L^-H -- for the corrections of the mixed form's complex64 phase
+ * (the orthonormalisation step of the reference's Davidson,
+ * qepy_pw/diagonalization.py:450-472, in Cholesky form).  *ok = 0 and
+ * ``out`` untouched when the Gram matrix is not positive definite or
+ * (max/min diag L)^2 exceeds ``condition`` (the caller then falls back to
+ * the canonical orthonormalisation).  Workspace: qepw_rr_workspace(nb). */
+int qepw_cholesky_qr(int precision, const void* block, int64_t ldb, int64_t npw, int nb,
+                     double condition, void* out, int64_t ldo, int* ok, void* workspace,
+                     size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

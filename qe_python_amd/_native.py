@@ -115,6 +115,8 @@ SIGNATURES = [
     ("qepw_rr_standardize", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                                     POINTER(c_int), c_void_p, c_size_t, c_void_p]),
     ("qepw_rr_spectral", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("qepw_cholesky_qr", c_int, [c_int, c_void_p, c_int64, c_int64, c_int, c_double, c_void_p,
+                                 c_int64, POINTER(c_int), c_void_p, c_size_t, c_void_p]),
 ]
 
 

@@ -804,6 +804,21 @@ int qepw_rr_spectral(const qepw_rr_spectral_args* args, double* values,
   return rc;
 }
 
+int qepw_cholesky_qr(int precision, const void* block, int64_t ldb, int64_t npw, int nb,
+                     double condition, void* out, int64_t ldo, int* ok, void* workspace,
+                     size_t workspace_bytes, void* stream) {
+  if ((precision != QEPW_C128 && precision != QEPW_C64) || !ok || nb < 0 || npw < 1 ||
+      ldb < npw || ldo < npw || (nb > 0 && (!block || !out || !workspace)) || block == out)
+    return fail(QEPW_ERR_ARG, "qepw_cholesky_qr: bad argument");
+  if (!qepw::host_lapack_ready())
+    return fail(QEPW_ERR_UNSUPPORTED,
+                "qepw_cholesky_qr: no host LAPACK registered (qepw_set_host_lapack)");
+  const int rc = qepw::cholesky_qr(precision, block, ldb, npw, nb, condition, out, ldo, ok,
+                                   workspace, workspace_bytes, static_cast<hipStream_t>(stream));
+  if (rc != QEPW_OK) g_error = "qepw_cholesky_qr: " + qepw::rr_message();
+  return rc;
+}
+
 int qepw_set_diagonal(void* a, int64_t lda, int n, const double* d, void* stream) {
   if (n < 0 || (n > 0 && (!a || lda < n)))
     return fail(QEPW_ERR_ARG, "qepw_set_diagonal: bad argument");

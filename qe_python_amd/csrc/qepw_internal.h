@@ -117,5 +117,8 @@ size_t rr_workspace_bytes(int maxsub);
 int rr_standardize(void* H, void* S, void* W, void* A, int maxsub, int first, int active,
                    int* not_pd, void* work, size_t work_bytes, hipStream_t stream);
 int rr_spectral(const qepw_rr_spectral_args* a, double* values, qepw_rr_spectral_result* out);
+int cholesky_qr(int precision, const void* block, int64_t ldb, int64_t npw, int nb,
+                double condition, void* out, int64_t ldo, int* ok, void* work,
+                size_t work_bytes, hipStream_t stream);
 
 }  // namespace qepw

@@ -475,4 +475,95 @@ int rr_spectral(const qepw_rr_spectral_args* a, double* values, qepw_rr_spectral
   }
 }
 
+// Cholesky QR of an (npw x nb) block of the given precision (see
+// _cholesky_orthonormalize): Gram = B^H B on the device, L L^H = Gram in FP64
+// on the host, out = B L^-H.  *ok = 0 (out not written) when the Gram
+// matrix is not positive definite or its squared condition estimate
+// (max/min of diag L)^2 exceeds ``condition``.
+int cholesky_qr(int precision, const void* block, int64_t ldb, int64_t npw, int nb,
+                double condition, void* out, int64_t ldo, int* ok, void* work,
+                size_t work_bytes, hipStream_t stream) {
+  *ok = 0;
+  try {
+    if (nb <= 0) {
+      *ok = 1;
+      return QEPW_OK;
+    }
+    const bool single = precision == QEPW_C64;
+    const size_t esz = single ? sizeof(ccplx) : sizeof(zcplx);
+    Carver cv{static_cast<char*>(work), work_bytes};
+    Ctx c{stream, nullptr, 0};
+    void* gram = cv.take((size_t)nb * nb * esz, "Gram");
+    void* tdev = cv.take((size_t)nb * nb * esz, "L^-H");
+    c.gemm_ws_bytes = cv.left > 256 ? cv.left - 256 : 0;
+    c.gemm_ws = cv.take(c.gemm_ws_bytes, "gemm");
+    const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+    check_rc(gemm(precision, QEPW_OP_C, nb, nb, npw, one, block, ldb, block, ldb, zero, gram, nb,
+                  c.gemm_ws, c.gemm_ws_bytes, stream),
+             "cholesky qr gemm");
+    std::vector<zcplx> m((size_t)nb * nb);
+    if (single) {
+      std::vector<ccplx> g32((size_t)nb * nb);
+      check_hip(t_read.reserve(g32.size() * sizeof(ccplx)), "hipHostMalloc");
+      check_hip(hipMemcpyAsync(t_read.ptr, gram, g32.size() * sizeof(ccplx),
+                               hipMemcpyDeviceToHost, stream),
+                "cholesky qr read");
+      check_hip(hipStreamSynchronize(stream), "cholesky qr wait");
+      t_upload[0].in_flight = t_upload[1].in_flight = false;
+      std::memcpy(g32.data(), t_read.ptr, g32.size() * sizeof(ccplx));
+      for (size_t i = 0; i < m.size(); ++i) m[i] = make_double2(g32[i].x, g32[i].y);
+    } else {
+      read_block(c, static_cast<zcplx*>(gram), nb, nb, nb, m.data());
+    }
+    // Hermitian part, lower triangle, and its Cholesky factor
+    for (int j = 0; j < nb; ++j)
+      for (int i = j; i < nb; ++i) {
+        const zcplx a = m[i + (size_t)j * nb], b = m[j + (size_t)i * nb];
+        m[i + (size_t)j * nb] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+      }
+    char lo = 'L', nd = 'N';
+    int nn = nb, ld = nb, info = 0;
+    g_lapack.zpotrf(&lo, &nn, m.data(), &ld, &info);
+    if (info != 0) return QEPW_OK;
+    double dmin = INFINITY, dmax = 0.0;
+    for (int i = 0; i < nb; ++i) {
+      const zcplx d = m[i + (size_t)i * nb];
+      const double a = std::hypot(d.x, d.y);
+      dmin = std::min(dmin, a);
+      dmax = std::max(dmax, a);
+    }
+    if (!(dmin > 0.0) || (dmax / dmin) * (dmax / dmin) > condition) return QEPW_OK;
+    g_lapack.ztrtri(&lo, &nd, &nn, m.data(), &ld, &info);
+    if (info != 0) return QEPW_OK;
+    // t = L^-H (upper triangular), column-major, in the block's precision
+    std::vector<zcplx> t((size_t)nb * nb, make_double2(0.0, 0.0));
+    for (int j = 0; j < nb; ++j)
+      for (int i = 0; i <= j; ++i) {
+        const zcplx v = m[j + (size_t)i * nb];
+        t[i + (size_t)j * nb] = make_double2(v.x, -v.y);
+      }
+    if (single) {
+      Pinned& p = t_upload[0];
+      if (p.in_flight) check_hip(hipStreamSynchronize(stream), "cholesky qr wait");
+      check_hip(p.reserve(t.size() * sizeof(ccplx)), "hipHostMalloc");
+      ccplx* h = static_cast<ccplx*>(p.ptr);
+      for (size_t i = 0; i < t.size(); ++i) h[i] = make_float2((float)t[i].x, (float)t[i].y);
+      check_hip(hipMemcpyAsync(tdev, p.ptr, t.size() * sizeof(ccplx), hipMemcpyHostToDevice,
+                               stream),
+                "cholesky qr upload");
+      p.in_flight = true;
+      p.stream = stream;
+    } else {
+      upload_block(c, 0, t.data(), nb, nb, static_cast<zcplx*>(tdev));
+    }
+    check_rc(gemm(precision, QEPW_OP_N, npw, nb, nb, one, block, ldb, tdev, nb, zero, out, ldo,
+                  c.gemm_ws, c.gemm_ws_bytes, stream),
+             "cholesky qr gemm");
+    *ok = 1;
+    return QEPW_OK;
+  } catch (const Fail& f) {
+    return f.code;
+  }
+}
+
 }  // namespace qepw

@@ -37,6 +37,7 @@ There is no CPU fallback: without a HIP device or without
 
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 from contextlib import nullcontext
 from dataclasses import dataclass
@@ -50,7 +51,7 @@ from . import _native as nat
 from . import device as dv
 from .eigensolver import DeviceRayleighRitz, _lower_inverse, _one_blas_thread
 from . import schedule
-from .schedule import Apply, Read, blocking
+from .schedule import Apply, Read, blocking, may_block
 
 # default reduced eigensolve of ``davidson``: "device" (GPU spectral
 # division) or "host" (LAPACK zhegvd, the reference's routine)
@@ -77,6 +78,9 @@ MIXED_LOCK_GAP = 1.0e-3     # Ha
 # the basis, before their Cholesky QR (on the medium workload a second pass
 # leaves the iteration count at 32 and costs ~4% of the solve)
 ORTHO_PASSES = 1
+# the complex64 corrections' Cholesky QR as one native call (qepw_cholesky_qr)
+# when the driver waits on reads itself; False keeps the Python steps
+NATIVE_ORTHO = True
 # subspace multiplier of the complex64 phase (None: the caller's
 # subspace_multiplier, as the FP64 refinement always uses)
 MIXED_COARSE_MULTIPLIER = None
@@ -615,6 +619,18 @@ def _cholesky_orthonormalize(block: torch.Tensor, npw: int, ws, condition: float
     canonical orthonormalisation."""
     n = int(block.shape[0])
     device = block.device
+    if NATIVE_ORTHO and may_block():
+        # the same steps in one native call (its host round trip inside)
+        lib = nat.load()
+        out = torch.empty_like(block)
+        buf = ws.get(int(lib.qepw_rr_workspace(max(n, 1))))
+        ok = ctypes.c_int(0)
+        nat.check(lib.qepw_cholesky_qr(dv.precision_code(block.dtype), block.data_ptr(),
+                                       dv.ld_of(block), npw, n, condition, out.data_ptr(),
+                                       dv.ld_of(out), ctypes.byref(ok), buf.data_ptr(),
+                                       buf.numel(), dv.stream_handle(device)),
+                  "qepw_cholesky_qr")
+        return out if ok.value else None
     gram_t = torch.empty((n, n), dtype=block.dtype, device=device)
     dv.gemm(nat.OP_C, n, n, npw, 1.0, block, block, 0.0, gram_t, ws)
     gram = (yield Read(gram_t)).T.astype(np.complex128)
