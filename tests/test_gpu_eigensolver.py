@@ -139,3 +139,33 @@ def test_dependent_basis_raises_like_zhegvd(native, monkeypatch):
     with pytest.raises(np.linalg.LinAlgError, match="zhegvd"):
         rr.solve(nb)
 
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.complex128, 1e-12), (torch.complex64, 2e-5)])
+def test_native_cholesky_qr_matches_python_steps(dtype, tol, monkeypatch):
+    """qepw_cholesky_qr (one native call) against the Python steps of the
+    same Cholesky QR: same orthonormal block, and both decline an
+    ill-conditioned block (the caller's cue for the canonical form)."""
+    from qe_python_amd import device as dv
+    from qe_python_amd import diagonalization as dg
+    from qe_python_amd.schedule import blocking
+    rng = np.random.default_rng(11)
+    npw, nb = 3000, 40
+    host = rng.standard_normal((npw, nb)) + 1j * rng.standard_normal((npw, nb))
+    block = dev_block(host).to(dtype)
+    ws = dv.Workspace(torch.device(DEV))
+    native = blocking(dg._cholesky_orthonormalize(block, npw, ws))
+    monkeypatch.setattr(dg, "NATIVE_ORTHO", False)
+    python = blocking(dg._cholesky_orthonormalize(block, npw, ws))
+    q = native.cpu().numpy().T.astype(np.complex128)
+    np.testing.assert_allclose(q, python.cpu().numpy().T, atol=tol * np.abs(q).max(), rtol=0)
+    np.testing.assert_allclose(q.conj().T @ q, np.eye(nb), atol=50 * tol)
+    # same span as the input
+    proj = q @ (q.conj().T @ host)
+    assert np.linalg.norm(proj - host) <= 50 * tol * np.linalg.norm(host)
+    bad = host.copy()
+    bad[:, 5] = bad[:, 4] + 1e-9 * bad[:, 6]
+    bad_block = dev_block(bad).to(dtype)
+    assert blocking(dg._cholesky_orthonormalize(bad_block, npw, ws)) is None
+    monkeypatch.setattr(dg, "NATIVE_ORTHO", True)
+    assert blocking(dg._cholesky_orthonormalize(bad_block, npw, ws)) is None
