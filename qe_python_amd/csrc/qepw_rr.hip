@@ -45,35 +45,58 @@ HostLapack g_lapack;
 
 thread_local std::string t_message;
 
-// Pinned host staging of one thread: a device-to-host read waits for the
-// stream, so a buffer read into is free again on return; uploads go from
-// slots of their own, and the stream is synchronised before a slot is
-// written again (every call of this file reads back at least once after
-// its uploads, so that wait has normally happened already).
+// Pinned host staging.  A device-to-host read waits for its stream, so a
+// buffer read into is free again on return; an upload records an event, and
+// its buffer is not written again before that event has completed.
 struct Pinned {
   void* ptr = nullptr;
   size_t bytes = 0;
   bool in_flight = false;
-  hipStream_t stream = nullptr;
-  ~Pinned() {
-    if (ptr) (void)hipHostFree(ptr);
+  hipEvent_t done = nullptr;
+  // the previous upload from this buffer has been copied
+  hipError_t settle() {
+    if (!in_flight) return hipSuccess;
+    in_flight = false;
+    return hipEventSynchronize(done);
   }
   hipError_t reserve(size_t need) {
-    if (need <= bytes) return hipSuccess;
-    if (ptr) {
-      if (in_flight && stream) (void)hipStreamSynchronize(stream);
-      (void)hipHostFree(ptr);
-      ptr = nullptr;
-      bytes = 0;
-    }
+    hipError_t e = settle();
+    if (e != hipSuccess || need <= bytes) return e;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
     const size_t grow = std::max(need, (size_t)1 << 20);
-    hipError_t e = hipHostMalloc(&ptr, grow, hipHostMallocDefault);
+    e = hipHostMalloc(&ptr, grow, hipHostMallocDefault);
     if (e == hipSuccess) bytes = grow;
-    in_flight = false;
     return e;
   }
 };
-thread_local Pinned t_read, t_upload[2];
+
+// One call's staging buffers, leased from a process-wide pool (calls from
+// several host threads each hold their own set; the sets live until the
+// process ends, so nothing is freed while the runtime shuts down).
+struct Staging {
+  Pinned read, up[2];
+};
+std::mutex g_staging_mutex;
+std::vector<Staging*> g_staging_free;
+
+struct StagingLease {
+  Staging* s;
+  StagingLease() {
+    std::lock_guard<std::mutex> lock(g_staging_mutex);
+    if (g_staging_free.empty()) {
+      s = new Staging();
+    } else {
+      s = g_staging_free.back();
+      g_staging_free.pop_back();
+    }
+  }
+  ~StagingLease() {
+    std::lock_guard<std::mutex> lock(g_staging_mutex);
+    g_staging_free.push_back(s);
+  }
+};
 
 struct Fail {
   int code;
@@ -116,6 +139,7 @@ struct Ctx {
   hipStream_t stream;
   void* gemm_ws;
   size_t gemm_ws_bytes;
+  Staging* st;
 };
 
 void gemm_z(const Ctx& c, int op, int64_t m, int64_t n, int64_t k, double alpha,
@@ -128,38 +152,43 @@ void gemm_z(const Ctx& c, int op, int64_t m, int64_t n, int64_t k, double alpha,
 }
 
 // device (rows x cols, ld) complex128 -> host column-major (rows x cols)
-void read_block(const Ctx& c, const zcplx* dev, int64_t ld, int rows, int cols, zcplx* host) {
-  const size_t bytes = (size_t)rows * cols * sizeof(zcplx);
-  check_hip(t_read.reserve(bytes), "hipHostMalloc");
-  check_hip(hipMemcpy2DAsync(t_read.ptr, rows * sizeof(zcplx), dev, ld * sizeof(zcplx),
-                             rows * sizeof(zcplx), cols, hipMemcpyDeviceToHost, c.stream),
+// device bytes -> host, through the call's read buffer (waits for the stream)
+void read_raw(const Ctx& c, const void* dev, int64_t ld_bytes, size_t row_bytes, int cols,
+              void* host) {
+  Pinned& p = c.st->read;
+  const size_t bytes = row_bytes * (size_t)cols;
+  check_hip(p.reserve(bytes), "hipHostMalloc");
+  check_hip(hipMemcpy2DAsync(p.ptr, row_bytes, dev, ld_bytes, row_bytes, cols,
+                             hipMemcpyDeviceToHost, c.stream),
             "rayleigh-ritz read");
   check_hip(hipStreamSynchronize(c.stream), "rayleigh-ritz wait");
-  t_upload[0].in_flight = t_upload[1].in_flight = false;
-  std::memcpy(host, t_read.ptr, bytes);
+  std::memcpy(host, p.ptr, bytes);
+}
+
+// device (rows x cols, ld) complex128 -> host column-major (rows x cols)
+void read_block(const Ctx& c, const zcplx* dev, int64_t ld, int rows, int cols, zcplx* host) {
+  read_raw(c, dev, ld * sizeof(zcplx), rows * sizeof(zcplx), cols, host);
 }
 
 void read_doubles(const Ctx& c, const double* dev, size_t count, double* host) {
-  const size_t bytes = count * sizeof(double);
-  check_hip(t_read.reserve(bytes), "hipHostMalloc");
-  check_hip(hipMemcpyAsync(t_read.ptr, dev, bytes, hipMemcpyDeviceToHost, c.stream),
-            "rayleigh-ritz read");
-  check_hip(hipStreamSynchronize(c.stream), "rayleigh-ritz wait");
-  t_upload[0].in_flight = t_upload[1].in_flight = false;
-  std::memcpy(host, t_read.ptr, bytes);
+  read_raw(c, dev, count * sizeof(double), count * sizeof(double), 1, host);
 }
 
-// host column-major (rows x cols) -> device (ld = rows), through upload slot s
-void upload_block(const Ctx& c, int s, const zcplx* host, int rows, int cols, zcplx* dev) {
-  Pinned& p = t_upload[s];
-  const size_t bytes = (size_t)rows * cols * sizeof(zcplx);
-  if (p.in_flight) check_hip(hipStreamSynchronize(c.stream), "rayleigh-ritz wait");
+// host bytes -> device, through upload slot s
+void upload_raw(const Ctx& c, int s, const void* host, size_t bytes, void* dev) {
+  Pinned& p = c.st->up[s];
   check_hip(p.reserve(bytes), "hipHostMalloc");
+  if (!p.done) check_hip(hipEventCreateWithFlags(&p.done, hipEventDisableTiming), "hipEventCreate");
   std::memcpy(p.ptr, host, bytes);
   check_hip(hipMemcpyAsync(dev, p.ptr, bytes, hipMemcpyHostToDevice, c.stream),
             "rayleigh-ritz upload");
+  check_hip(hipEventRecord(p.done, c.stream), "hipEventRecord");
   p.in_flight = true;
-  p.stream = c.stream;
+}
+
+// host column-major (rows x cols) -> device (ld = rows)
+void upload_block(const Ctx& c, int s, const zcplx* host, int rows, int cols, zcplx* dev) {
+  upload_raw(c, s, host, (size_t)rows * cols * sizeof(zcplx), dev);
 }
 
 // lower Cholesky factor L of the Hermitian part of column-major m (n x n,
@@ -266,7 +295,8 @@ int rr_standardize(void* H, void* S, void* W, void* A, int maxsub, int first, in
     const int nnew = active - first;
     if (nnew <= 0) return QEPW_OK;
     Carver cv{static_cast<char*>(work), work_bytes};
-    Ctx c{stream, nullptr, 0};
+    StagingLease lease;
+    Ctx c{stream, nullptr, 0, lease.s};
     const size_t z = sizeof(zcplx);
     zcplx* Z = static_cast<zcplx*>(cv.take((size_t)nnew * std::max(1, first) * z, "Z"));
     zcplx* M = static_cast<zcplx*>(cv.take((size_t)nnew * nnew * z, "M"));
@@ -329,7 +359,8 @@ int rr_spectral(const qepw_rr_spectral_args* a, double* values, qepw_rr_spectral
     const int64_t ld = a->maxsub;
     hipStream_t stream = static_cast<hipStream_t>(a->stream);
     Carver cv{static_cast<char*>(a->work), a->work_bytes};
-    Ctx c{stream, nullptr, 0};
+    StagingLease lease;
+    Ctx c{stream, nullptr, 0, lease.s};
     const size_t sq = (size_t)a->maxsub * a->maxsub * sizeof(zcplx);
     zcplx* Om = static_cast<zcplx*>(cv.take(sq, "Omega"));
     zcplx* Q0 = static_cast<zcplx*>(cv.take(sq, "Q0"));
@@ -492,7 +523,8 @@ int cholesky_qr(int precision, const void* block, int64_t ldb, int64_t npw, int 
     const bool single = precision == QEPW_C64;
     const size_t esz = single ? sizeof(ccplx) : sizeof(zcplx);
     Carver cv{static_cast<char*>(work), work_bytes};
-    Ctx c{stream, nullptr, 0};
+    StagingLease lease;
+    Ctx c{stream, nullptr, 0, lease.s};
     void* gram = cv.take((size_t)nb * nb * esz, "Gram");
     void* tdev = cv.take((size_t)nb * nb * esz, "L^-H");
     c.gemm_ws_bytes = cv.left > 256 ? cv.left - 256 : 0;
@@ -504,13 +536,8 @@ int cholesky_qr(int precision, const void* block, int64_t ldb, int64_t npw, int 
     std::vector<zcplx> m((size_t)nb * nb);
     if (single) {
       std::vector<ccplx> g32((size_t)nb * nb);
-      check_hip(t_read.reserve(g32.size() * sizeof(ccplx)), "hipHostMalloc");
-      check_hip(hipMemcpyAsync(t_read.ptr, gram, g32.size() * sizeof(ccplx),
-                               hipMemcpyDeviceToHost, stream),
-                "cholesky qr read");
-      check_hip(hipStreamSynchronize(stream), "cholesky qr wait");
-      t_upload[0].in_flight = t_upload[1].in_flight = false;
-      std::memcpy(g32.data(), t_read.ptr, g32.size() * sizeof(ccplx));
+      read_raw(c, gram, (int64_t)g32.size() * sizeof(ccplx), g32.size() * sizeof(ccplx), 1,
+               g32.data());
       for (size_t i = 0; i < m.size(); ++i) m[i] = make_double2(g32[i].x, g32[i].y);
     } else {
       read_block(c, static_cast<zcplx*>(gram), nb, nb, nb, m.data());
@@ -543,16 +570,9 @@ int cholesky_qr(int precision, const void* block, int64_t ldb, int64_t npw, int 
         t[i + (size_t)j * nb] = make_double2(v.x, -v.y);
       }
     if (single) {
-      Pinned& p = t_upload[0];
-      if (p.in_flight) check_hip(hipStreamSynchronize(stream), "cholesky qr wait");
-      check_hip(p.reserve(t.size() * sizeof(ccplx)), "hipHostMalloc");
-      ccplx* h = static_cast<ccplx*>(p.ptr);
-      for (size_t i = 0; i < t.size(); ++i) h[i] = make_float2((float)t[i].x, (float)t[i].y);
-      check_hip(hipMemcpyAsync(tdev, p.ptr, t.size() * sizeof(ccplx), hipMemcpyHostToDevice,
-                               stream),
-                "cholesky qr upload");
-      p.in_flight = true;
-      p.stream = stream;
+      std::vector<ccplx> t32(t.size());
+      for (size_t i = 0; i < t.size(); ++i) t32[i] = make_float2((float)t[i].x, (float)t[i].y);
+      upload_raw(c, 0, t32.data(), t32.size() * sizeof(ccplx), tdev);
     } else {
       upload_block(c, 0, t.data(), nb, nb, static_cast<zcplx*>(tdev));
     }
