@@ -98,6 +98,15 @@ int qepw_apply_local(const qepw_plan* plan, int precision, const void* vectors,
                      int64_t ldv, int nb, const void* potential, const void* kinetic,
                      void* result, int64_t ldr, void* workspace, size_t ws_bytes,
                      void* stream);
+/* qepw_apply_local_timed: qepw_apply_local with the HIP events start_event /
+ * stop_event (hipEvent_t) recorded on ``stream`` immediately before its first
+ * and after its last kernel, so that their elapsed time spans the FFT phase's
+ * kernels and not the caller's launch latency (the measurement of the phase's
+ * HBM roofline; no counterpart in the reference). */
+int qepw_apply_local_timed(const qepw_plan* plan, int precision, const void* vectors,
+                           int64_t ldv, int nb, const void* potential, const void* kinetic,
+                           void* result, int64_t ldr, void* workspace, size_t ws_bytes,
+                           void* start_event, void* stop_event, void* stream);
 
 /*
  * qepw_apply_local for ``count`` independent problems at once (k-points or

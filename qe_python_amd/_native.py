@@ -41,6 +41,9 @@ SIGNATURES = [
     ("qepw_apply_local_workspace", c_size_t, [c_void_p, c_int, c_int]),
     ("qepw_apply_local", c_int, [c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p, c_void_p,
                                  c_void_p, c_int64, c_void_p, c_size_t, c_void_p]),
+    ("qepw_apply_local_timed", c_int, [c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p,
+                                       c_void_p, c_void_p, c_int64, c_void_p, c_size_t, c_void_p,
+                                       c_void_p, c_void_p]),
     ("qepw_apply_local_group", c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                        c_void_p, c_void_p]),

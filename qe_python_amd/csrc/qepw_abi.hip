@@ -342,6 +342,22 @@ int qepw_apply_local(const qepw_plan* plan, int precision, const void* vectors, 
   return finish(rc, "qepw_apply_local");
 }
 
+int qepw_apply_local_timed(const qepw_plan* plan, int precision, const void* vectors,
+                           int64_t ldv, int nb, const void* potential, const void* kinetic,
+                           void* result, int64_t ldr, void* workspace, size_t ws_bytes,
+                           void* start_event, void* stop_event, void* stream) {
+  if (!start_event || !stop_event) return fail(QEPW_ERR_ARG, "qepw_apply_local_timed: NULL event");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipEventRecord(static_cast<hipEvent_t>(start_event), s);
+  if (e != hipSuccess) return fail_hip(e, "qepw_apply_local_timed: hipEventRecord");
+  const int rc = qepw_apply_local(plan, precision, vectors, ldv, nb, potential, kinetic, result,
+                                  ldr, workspace, ws_bytes, stream);
+  if (rc != QEPW_OK) return rc;
+  e = hipEventRecord(static_cast<hipEvent_t>(stop_event), s);
+  if (e != hipSuccess) return fail_hip(e, "qepw_apply_local_timed: hipEventRecord");
+  return QEPW_OK;
+}
+
 int qepw_apply_local_group(int count, const qepw_plan* const* plans, int precision,
                            const void* const* vectors, const int64_t* ldv, const int* nb,
                            const void* const* potentials, const void* const* kinetics,

@@ -418,6 +418,27 @@ class DeviceFFTPlan:
             buf.data_ptr(), min(buf.numel(), need), stream_handle(vectors.device))
         nat.check(rc, "qepw_apply_local")
 
+    def apply_local_timed(self, vectors: torch.Tensor, result: torch.Tensor,
+                          potential: torch.Tensor, kinetic: torch.Tensor | None, ws: Workspace,
+                          start: "torch.cuda.Event", stop: "torch.cuda.Event") -> None:
+        """``apply_local`` with ``start`` / ``stop`` (torch timing events,
+        already recorded once so that their HIP events exist) recorded by the
+        library right around the phase's kernels."""
+        nb = vectors.shape[0]
+        if nb == 0:
+            return
+        prec = precision_code(vectors.dtype)
+        if result.dtype != vectors.dtype or potential.dtype != real_dtype(vectors.dtype):
+            raise ValueError("local-potential operands disagree in precision")
+        need = self.workspace_bytes(vectors.dtype, min(nb, 256))
+        buf = ws.get(need)
+        rc = nat.load().qepw_apply_local_timed(
+            self.handle, prec, vectors.data_ptr(), ld_of(vectors), nb, potential.data_ptr(),
+            None if kinetic is None else kinetic.data_ptr(), result.data_ptr(), ld_of(result),
+            buf.data_ptr(), min(buf.numel(), need), start.cuda_event, stop.cuda_event,
+            stream_handle(vectors.device))
+        nat.check(rc, "qepw_apply_local_timed")
+
     @staticmethod
     def apply_local_group(items) -> None:
         """``apply_local`` of several plans at once, ``items`` a sequence of

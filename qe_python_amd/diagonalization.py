@@ -419,12 +419,15 @@ class PlaneWaveHamiltonian:
             if self.fft_events is None:
                 self.plan.apply_local(src, dst, res.potential, res.kinetic, self.workspace)
             else:
-                # HIP events on the launch stream around the FFT phase
+                # HIP events on the launch stream, recorded by the library
+                # right around the FFT phase's kernels (recorded once here
+                # first, so that torch has created them)
                 start = torch.cuda.Event(enable_timing=True)
                 stop = torch.cuda.Event(enable_timing=True)
                 start.record()
-                self.plan.apply_local(src, dst, res.potential, res.kinetic, self.workspace)
                 stop.record()
+                self.plan.apply_local_timed(src, dst, res.potential, res.kinetic,
+                                            self.workspace, start, stop)
                 self.fft_events.append((nb, src.dtype, start, stop))
         if res.beta is not None:
             self._nonlocal_apply(src, dst, res)

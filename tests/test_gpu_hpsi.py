@@ -299,3 +299,28 @@ def test_grouped_apply_equals_single_apply():
             assert torch.equal(got, one), (ham.plan.shape, dtype)
             assert ham.applications == b0 + nb
     PlaneWaveHamiltonian.device_apply_group([])
+
+
+def test_timed_apply_matches_and_times_the_phase(medium):
+    """qepw_apply_local_timed: the same result as qepw_apply_local, and its
+    events span the FFT phase's kernels (positive, below the span of a
+    Python-level event pair around the same call)."""
+    ham = device_ham(medium)
+    x = ham.to_device(medium.trial[:, :64])
+    res = ham._resources(x.dtype)
+    ref = torch.empty_like(x)
+    ham.plan.apply_local(x, ref, res.potential, res.kinetic, ham.workspace)
+    got = torch.full_like(x, float("nan"))
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    stop.record()
+    outer0 = torch.cuda.Event(enable_timing=True)
+    outer1 = torch.cuda.Event(enable_timing=True)
+    outer0.record()
+    ham.plan.apply_local_timed(x, got, res.potential, res.kinetic, ham.workspace, start, stop)
+    outer1.record()
+    torch.cuda.synchronize()
+    assert torch.equal(got, ref)
+    inner = start.elapsed_time(stop)
+    assert 0.0 < inner <= outer0.elapsed_time(outer1) + 1e-3
