@@ -71,7 +71,7 @@ def parse():
     ap.add_argument("--no-legs", action="store_true",
                     help="skip the density, projector, iterative-solver and "
                          "wavefunction-file legs (the Davidson solves only)")
-    ap.add_argument("--batch-workers", type=int, default=2)
+    ap.add_argument("--batch-workers", type=int, default=3)
     ap.add_argument("--breakdown", action="store_true",
                     help="print a per-label timer breakdown of one extra solve to stderr")
     ap.add_argument("--fft-probe", type=int, default=0, metavar="CALLS",

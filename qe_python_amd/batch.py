@@ -14,8 +14,10 @@ schedules:
   the host side of each solve -- the reduced eigensolve's native calls
   (``qepw_rr_standardize`` / ``qepw_rr_spectral``, which release the GIL),
   its LAPACK on a single-threaded BLAS -- runs while the other threads
-  launch.  Two workers measured best on one MI355X with the 24-problem
-  uneven workload (1.5x over one by one; more contend for the GIL).
+  launch.  Three workers measured best on one MI355X with the 24-problem
+  uneven workload (1.8x over one by one, 15% over two): a process gets four
+  hardware queues, one of them the default stream's, so a fourth worker
+  stream shares a queue with another and its kernels wait behind them.
 * ``"rounds"``: every problem's Davidson is the generator
   ``davidson_steps``; ``schedule.run_many`` advances all of them in lock-step
   rounds on one host thread and one stream.  In a round, H is applied to
@@ -81,7 +83,7 @@ def davidson_batch(problems, *, workers: int | None = None, device=None,
 
     ``common`` holds ``davidson`` keywords shared by all problems (a
     problem's ``options`` override them).  ``schedule`` is "threads"
-    (``workers`` host threads, default 2, each driving one problem at a time
+    (``workers`` host threads, default 3, each driving one problem at a time
     on its own stream) or "rounds" (all problems in lock-step rounds with
     grouped H applications, on the current stream).  Device tensors
     returned with ``device_output=True`` are complete when this returns.
@@ -122,7 +124,7 @@ def _batch_rounds(problems, dev, common, stats) -> list[DavidsonResult]:
 
 
 def _batch_threads(problems, dev, workers, common) -> list[DavidsonResult]:
-    workers = max(1, min(len(problems), 2 if workers is None else int(workers)))
+    workers = max(1, min(len(problems), 3 if workers is None else int(workers)))
     streams = _worker_streams(dev, workers)
     # operands uploaded on the caller's stream must be complete before the
     # worker streams read them

@@ -413,12 +413,18 @@ __device__ __forceinline__ void planes_team_body(
       const int* pos = colpos + c * N;
       cplx<T> bb[L], a[M];
       if (t < M) {
+        // every position first, then every stick load (from element 0 where
+        // the slot holds no stick, replaced by zero): two memory round trips
+        // per column instead of two per slot
+        int st[L];
+#pragma unroll
+        for (int s = 0; s < L; ++s) st[s] = ((KEEP >> s) & 1u) ? pos[t + M * s] : -1;
 #pragma unroll
         for (int s = 0; s < L; ++s) {
           bb[s] = zero;
           if ((KEEP >> s) & 1u) {
-            const int st = pos[t + M * s];
-            if (st >= 0) bb[s] = w[st];
+            const cplx<T> v = w[st[s] >= 0 ? st[s] : 0];
+            bb[s] = st[s] >= 0 ? v : zero;
           }
         }
       }
@@ -494,6 +500,11 @@ __device__ __forceinline__ void planes_team_body(
       cplx<T>* col = Q + c * NP;
       const int* pos = colpos + c * N;
       cplx<T> a[M], bb[L];
+      // the stick positions are fetched before the transform, so that the
+      // stores below wait for nothing
+      int st[L];
+#pragma unroll
+      for (int k = 0; k < L; ++k) st[k] = ((KEEP >> k) & 1u) ? pos[t + M * k] : -1;
       if (t < L) {
 #pragma unroll
         for (int s = 0; s < M; ++s) a[s] = col[t + L * s];
@@ -503,13 +514,14 @@ __device__ __forceinline__ void planes_team_body(
       wave_lds_sync();
       dit_back<T, N, -1>(t, col, bb);
       if (t < M) {
+        // every position is consumed before the first store: one wait for
+        // the loads above, none between the stores (stores count in vmcnt)
 #pragma unroll
-        for (int k = 0; k < L; ++k) {
-          if ((KEEP >> k) & 1u) {
-            const int st = pos[t + M * k];
-            if (st >= 0) w[st] = bb[k];
-          }
-        }
+        for (int k = 0; k < L; ++k)
+          if ((KEEP >> k) & 1u) asm volatile("" ::"v"(st[k]));
+#pragma unroll
+        for (int k = 0; k < L; ++k)
+          if (((KEEP >> k) & 1u) && st[k] >= 0) w[st[k]] = bb[k];
       }
     }
   }
