@@ -439,8 +439,10 @@ int gemm_plan(int precision, int op_a, int64_t m, int64_t n, int64_t k, int64_t*
 // (k, j) is read as conj(Y[j, k]), a contiguous run like X's (i, k).  The four
 // waves of a workgroup split K, each fetching its next kHermUnroll MFMA
 // steps while the current ones run, and their partial tiles are summed in LDS
-// in wave order, so results are reproducible.
-constexpr int kHermUnroll = 8;
+// in wave order, so results are reproducible.  Two steps in flight measured
+// fastest (n = 264 / 432 / 512: 7.1 / 15.2 / 21.4 us against 8.4 / 18.3 /
+// 26.3 us with eight; four and six in between).
+constexpr int kHermUnroll = 2;
 
 __global__ __launch_bounds__(256) void k_herm_product(
     int n, const double2* __restrict__ X, const double2* __restrict__ Y,
