@@ -15,9 +15,10 @@ schedules:
   (``qepw_rr_standardize`` / ``qepw_rr_spectral``, which release the GIL),
   its LAPACK on a single-threaded BLAS -- runs while the other threads
   launch.  Three workers measured best on one MI355X with the 24-problem
-  uneven workload (1.8x over one by one, 15% over two): a process gets four
-  hardware queues, one of them the default stream's, so a fourth worker
-  stream shares a queue with another and its kernels wait behind them.
+  uneven workload (1.8x over one by one, 15% over two); more are slower
+  with four hardware queues per process and with eight alike, so what
+  limits them is the host: the Python parts of the solves take turns
+  holding the GIL.
 * ``"rounds"``: every problem's Davidson is the generator
   ``davidson_steps``; ``schedule.run_many`` advances all of them in lock-step
   rounds on one host thread and one stream.  In a round, H is applied to
