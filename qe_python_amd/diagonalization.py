@@ -961,7 +961,17 @@ def _davidson_phase_body(ctx, start, dtype, energy_bounds, residual_bounds, resi
         basis = start[:number_of_roots]
     else:
         with _measure(timers, "cegterg:over", device):
-            basis = yield from _orthonormalize_steps(start, npw, ws)
+            # Cholesky QR (one small factorisation) spans the same space as the
+            # canonical orthonormalisation when the block is well conditioned,
+            # and the Rayleigh-Ritz depends only on that span; the canonical
+            # form (a host eigensolve, dropping dependent directions) takes
+            # the rest
+            basis = None
+            if start.shape[0] > 1:
+                basis = yield from _cholesky_orthonormalize(
+                    start, npw, ws, condition=1.0e6 if dtype == torch.complex64 else 1.0e8)
+            if basis is None:
+                basis = yield from _orthonormalize_steps(start, npw, ws)
     if basis.shape[0] < number_of_roots:
         raise ValueError("initial Davidson vectors are linearly dependent")
     maximum_subspace = min(ctx.dimension, max(basis.shape[0], 2 * number_of_roots,
