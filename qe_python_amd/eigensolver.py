@@ -283,17 +283,18 @@ class DeviceRayleighRitz:
         # and the spectral solve run as one native call each (their host
         # round trips inside the library); under run_many they yield
         native = NATIVE_RR and may_block()
+        spectral = self.theta is not None and active > k + 8
+        out = None
         if native:
+            prepared = self._spectral_native_prepare(active) if spectral else None
             self._standardize_native(active)
+            if spectral:
+                out = self._spectral_native_run(prepared)
         else:
             yield from self._standardize(active)
-        out = None
-        if self.theta is not None and active > k + 8:
-            if native:
-                out = self._spectral_native(active)
-            else:
+            if spectral:
                 out = yield from self._spectral(active)
-        else:
+        if not spectral:
             self.stats["skipped"] += 1
         if out is not None:
             self.stats["spectral"] += 1
@@ -444,6 +445,13 @@ class DeviceRayleighRitz:
     def _spectral_native(self, n):
         """``_spectral`` with the sign iteration, the range sample and the
         Rayleigh-Ritz on it as one native call (qepw_rr_spectral)."""
+        return self._spectral_native_run(self._spectral_native_prepare(n))
+
+    def _spectral_native_prepare(self, n):
+        """The shift and the arguments of the native spectral solve, or None
+        when no shift can be placed.  They depend only on the previous Ritz
+        values, so they are ready before the standardisation runs and the
+        device does not wait for them between the two calls."""
         k = self.nroots
         shift = self._choose_shift()
         if shift is None:
@@ -464,6 +472,15 @@ class DeviceRayleighRitz:
             gauss=_gaussian_columns(self.maxsub, self.device).data_ptr(),
             coef=coef.data_ptr(), rb=self.rb.data_ptr(), work=buf.data_ptr(),
             work_bytes=buf.numel(), stream=dv.stream_handle(self.device))
+        # buf stays referenced with the arguments that point into it
+        return args, coef, sigma, have_prev, buf
+
+    def _spectral_native_run(self, prepared):
+        if prepared is None:
+            return None
+        args, coef, sigma, have_prev, _ = prepared
+        k = self.nroots
+        lib = nat.load()
         values = np.empty(self.maxsub, dtype=np.float64)
         res = nat.RRSpectralResult()
         nat.check(lib.qepw_rr_spectral(ctypes.byref(args), values.ctypes.data,
