@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <mutex>
 #include <new>
 #include <string>
@@ -128,8 +129,8 @@ const char* qepw_last_error(void) { return g_error.c_str(); }
 
 int qepw_abi_version(void) { return 1; }
 
-int qepw_plan_create(const int32_t* miller, int64_t npw, int n1, int n2, int n3, int device,
-                     qepw_plan** out) {
+static int plan_create_impl(const int32_t* miller, int64_t npw, int n1, int n2, int n3,
+                            int device, qepw_plan** out) {
   if (!out) return fail(QEPW_ERR_ARG, "qepw_plan_create: out is NULL");
   *out = nullptr;
   if (!miller || npw < 1) return fail(QEPW_ERR_ARG, "qepw_plan_create: empty basis");
@@ -262,6 +263,17 @@ int qepw_plan_create(const int32_t* miller, int64_t npw, int n1, int n2, int n3,
   return QEPW_OK;
 }
 
+// host allocations of the tables can throw: no exception leaves the C ABI
+int qepw_plan_create(const int32_t* miller, int64_t npw, int n1, int n2, int n3, int device,
+                     qepw_plan** out) {
+  try {
+    return plan_create_impl(miller, npw, n1, n2, n3, device, out);
+  } catch (const std::exception& e) {
+    if (out) *out = nullptr;
+    return fail(QEPW_ERR_HIP, (std::string("qepw_plan_create: host allocation: ") + e.what()).c_str());
+  }
+}
+
 int qepw_plan_destroy(qepw_plan* plan) {
   if (!plan) return QEPW_OK;
   int prev = 0;
@@ -358,11 +370,12 @@ int qepw_apply_local_timed(const qepw_plan* plan, int precision, const void* vec
   return QEPW_OK;
 }
 
-int qepw_apply_local_group(int count, const qepw_plan* const* plans, int precision,
-                           const void* const* vectors, const int64_t* ldv, const int* nb,
-                           const void* const* potentials, const void* const* kinetics,
-                           void* const* results, const int64_t* ldr, void* const* workspaces,
-                           const size_t* ws_bytes, void* stream) {
+static int apply_local_group_impl(int count, const qepw_plan* const* plans, int precision,
+                                  const void* const* vectors, const int64_t* ldv,
+                                  const int* nb, const void* const* potentials,
+                                  const void* const* kinetics, void* const* results,
+                                  const int64_t* ldr, void* const* workspaces,
+                                  const size_t* ws_bytes, void* stream) {
   if (count < 0 || (count > 0 && (!plans || !vectors || !ldv || !nb || !potentials ||
                                   !kinetics || !results || !ldr || !workspaces || !ws_bytes)))
     return fail(QEPW_ERR_ARG, "qepw_apply_local_group: NULL argument");
@@ -381,6 +394,20 @@ int qepw_apply_local_group(int count, const qepw_plan* const* plans, int precisi
                                         potentials, kinetics, results, ldr, workspaces,
                                         ws_bytes, static_cast<hipStream_t>(stream)),
                 "qepw_apply_local_group");
+}
+
+int qepw_apply_local_group(int count, const qepw_plan* const* plans, int precision,
+                           const void* const* vectors, const int64_t* ldv, const int* nb,
+                           const void* const* potentials, const void* const* kinetics,
+                           void* const* results, const int64_t* ldr, void* const* workspaces,
+                           const size_t* ws_bytes, void* stream) {
+  try {
+    return apply_local_group_impl(count, plans, precision, vectors, ldv, nb, potentials,
+                                  kinetics, results, ldr, workspaces, ws_bytes, stream);
+  } catch (const std::exception& e) {
+    return fail(QEPW_ERR_HIP,
+                (std::string("qepw_apply_local_group: host allocation: ") + e.what()).c_str());
+  }
 }
 
 size_t qepw_inverse_real_workspace(const qepw_plan* plan) {

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -345,6 +346,10 @@ int rr_standardize(void* H, void* S, void* W, void* A, int maxsub, int first, in
     return QEPW_OK;
   } catch (const Fail& f) {
     return f.code;
+  } catch (const std::exception& e) {
+    // host allocations (staging, LAPACK work arrays)
+    t_message = std::string("host allocation: ") + e.what();
+    return QEPW_ERR_HIP;
   }
 }
 
@@ -503,6 +508,10 @@ int rr_spectral(const qepw_rr_spectral_args* a, double* values, qepw_rr_spectral
     return QEPW_OK;
   } catch (const Fail& f) {
     return f.code;
+  } catch (const std::exception& e) {
+    // host allocations (staging, LAPACK work arrays)
+    t_message = std::string("host allocation: ") + e.what();
+    return QEPW_ERR_HIP;
   }
 }
 
@@ -583,6 +592,10 @@ int cholesky_qr(int precision, const void* block, int64_t ldb, int64_t npw, int 
     return QEPW_OK;
   } catch (const Fail& f) {
     return f.code;
+  } catch (const std::exception& e) {
+    // host allocations (staging, LAPACK work arrays)
+    t_message = std::string("host allocation: ") + e.what();
+    return QEPW_ERR_HIP;
   }
 }
 

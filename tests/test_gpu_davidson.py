@@ -66,6 +66,34 @@ def test_small_seven_roots(small):
     assert np.all(res.residual_norms < 10 * TOL_HA)
 
 
+@pytest.mark.parametrize("precision,separation", [("fp64", 1e-5), ("fp64", 1e-2),
+                                                   ("mixed", 1e-2)])
+def test_nearly_dependent_trials(small, precision, separation, monkeypatch):
+    """A trial block with one column close to another: a Gram condition above
+    the Cholesky QR's limit sends the initial orthonormalisation to the
+    canonical form (1e-5 in FP64), below it Cholesky QR is taken; the
+    eigenvalues are the dense ones either way."""
+    import qe_python_amd.diagonalization as dg
+    s, ham, exact = small
+    canonical = []
+    original = dg._orthonormalize_steps
+
+    def counting(*args, **kwargs):
+        canonical.append(int(args[0].shape[0]))
+        return (yield from original(*args, **kwargs))
+
+    monkeypatch.setattr(dg, "_orthonormalize_steps", counting)
+    trial = np.array(s.trial, dtype=np.complex128, order="F")
+    trial[:, 5] = trial[:, 0] + separation * trial[:, 5]
+    res = dg.davidson(ham.apply, ham.diagonal, 7, initial_vectors=trial, tolerance=TOL_HA,
+                      operator_into=ham.apply_into, precision=precision)
+    assert res.converged
+    np.testing.assert_allclose(res.eigenvalues, exact[:7], atol=TOL_HA, rtol=0)
+    if precision == "fp64":
+        # the FP64 form orthonormalises only the initial block
+        assert canonical == ([trial.shape[1]] if separation < 1e-4 else [])
+
+
 def test_small_sixteen_roots_without_residual_test(small):
     from qe_python_amd.diagonalization import davidson
     s, ham, exact = small
