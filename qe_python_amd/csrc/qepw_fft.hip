@@ -1124,7 +1124,8 @@ int apply_local_group(int count, const Plan* const* plans, int precision,
 // ---- density accumulation -------------------------------------------------
 // rho(r) = scale * sum_b w_b |psi_b(r)|^2 with psi_b(r) the unnormalised
 // backward FFT (reference: accumulate_density_serial, _native_fft.pyx:
-// 2592-2648).  The backward z stage is k_sticks_inverse; the plane stage
+// 2592-2648).  The backward z stage is apply_local's (the team kernel where
+// n3 has one, k_sticks_inverse otherwise); the plane stage
 // loops over a fixed slice of bands inside each workgroup and accumulates in
 // LDS / registers, writing one partial plane per (slice, z); a last kernel
 // sums the slices in a fixed order.  No atomics: results are reproducible.
@@ -1132,7 +1133,7 @@ int apply_local_group(int count, const Plan* const* plans, int precision,
 template <typename T, int N, bool PRUNE>
 __global__ __launch_bounds__(kTeamThreads) void k_density_team(
     const cplx<T>* __restrict__ work, int64_t band_stride, int nst,
-    const int* __restrict__ stick_q, const int* __restrict__ colmap, int nxc, int nb,
+    const int* __restrict__ colpos, const int* __restrict__ colmap, int nxc, int nb,
     int bands_per_slice, const double* __restrict__ weights, double* __restrict__ partial,
     int accumulate, const cplx<T>* __restrict__ tw) {
   using G = TeamGeom<N>;
@@ -1156,7 +1157,6 @@ __global__ __launch_bounds__(kTeamThreads) void k_density_team(
   const int slice = blockIdx.y;
   const int b0 = slice * bands_per_slice;
   const int b1 = min(nb, b0 + bands_per_slice);
-  const int nq = (nxc + 1) * NP;
   const cplx<T> zero = mk<T>(T(0), T(0));
   TeamTwiddles<T, N> twr;
   twr.load(t, tw);
@@ -1171,25 +1171,34 @@ __global__ __launch_bounds__(kTeamThreads) void k_density_team(
   for (int r = 0; r < R; ++r)
 #pragma unroll
     for (int k = 0; k < M; ++k) acc[r][k] = 0.0;
+  // the zero column, read by the row pass at the inactive x positions (the
+  // column pass below never writes it; its closing barrier orders this)
+  for (int i = tid; i < NP; i += kTeamThreads) Q[nxc * NP + i] = zero;
   for (int b = b0; b < b1; ++b) {
     const cplx<T>* w = work + (int64_t)b * band_stride + (int64_t)z * nst;
     const double wb = weights[b];
-    for (int i = tid; i < nq; i += kTeamThreads) Q[i] = zero;
-    __syncthreads();
-    for (int s = tid; s < nst; s += kTeamThreads) {
-      const int q = stick_q[s];
-      Q[q + q / N] = w[s];
-    }
-    __syncthreads();
-    // backward y-FFT of every active column (in place), inputs pruned
+    // backward y-FFT of every active column, its inputs read straight from
+    // the stick buffer (colpos: the stick at (c, y), or -1; every position of
+    // a column first, then every stick load), its outputs written to the
+    // column
     if (in_team) {
       for (int c = team; c < nxc; c += G::teams) {
         if (c != team) wave_lds_sync();
         cplx<T>* col = Q + c * NP;
+        const int* pos = colpos + c * N;
         cplx<T> bb[L], a[M];
         if (t < M) {
+          int st[L];
 #pragma unroll
-          for (int s = 0; s < L; ++s) bb[s] = ((KEEP >> s) & 1u) ? col[t + M * s] : zero;
+          for (int s = 0; s < L; ++s) st[s] = ((KEEP >> s) & 1u) ? pos[t + M * s] : -1;
+#pragma unroll
+          for (int s = 0; s < L; ++s) {
+            bb[s] = zero;
+            if ((KEEP >> s) & 1u) {
+              const cplx<T> v = w[st[s] >= 0 ? st[s] : 0];
+              bb[s] = st[s] >= 0 ? v : zero;
+            }
+          }
         }
         dif_front<T, N, +1>(t, S, twr, bb);
         wave_lds_sync();
@@ -1384,7 +1393,8 @@ int accumulate_density_t(const Plan& p, const void* vectors, int64_t ldv, int nb
   const size_t np = (size_t)p.n1 * p.n2;
   if (st.group * p.n3 > lds_fft_capacity(p.line[2], kStickThreads, kStickPts))
     return QEPW_ERR_UNSUPPORTED;
-  const size_t team_lds = (p.n1 == p.n2 && st.d_stickq) ? density_team_lds<T>(p.n1, st.nxcols) : 0;
+  const size_t team_lds =
+      (p.n1 == p.n2 && st.d_colpos && st.d_colmap) ? density_team_lds<T>(p.n1, st.nxcols) : 0;
   const bool team = team_lds > 0 && team_lds <= (size_t)kMaxLdsBytes;
   // planes that neither the team kernel nor one workgroup's LDS can hold go
   // through the split stage's column lines, as in apply_local
@@ -1429,9 +1439,12 @@ int accumulate_density_t(const Plan& p, const void* vectors, int64_t ldv, int nb
   for (int b0 = 0; b0 < nb; b0 += chunk) {
     const int nbc = (nb - b0) < chunk ? (nb - b0) : chunk;
     const cplx<T>* src = static_cast<const cplx<T>*>(vectors) + (int64_t)b0 * ldv;
-    hipLaunchKernelGGL(k_sticks_inverse<T>, dim3(st.ngroups, nbc), dim3(kStickThreads),
-                       stick_lds, stream, src, ldv, work, bstride, st.nst, p.n3, st.d_off,
-                       st.d_gpos, st.group, p.line[2], twiddles<T>(p, 2));
+    // the z stage of apply_local: the team kernel where n3 has one
+    if (!launch_sticks_team<T>(true, p, st, src, ldv, work, bstride, nullptr, nullptr, 0,
+                               twiddles<T>(p, 2), T(0), nbc, stream))
+      hipLaunchKernelGGL(k_sticks_inverse<T>, dim3(st.ngroups, nbc), dim3(kStickThreads),
+                         stick_lds, stream, src, ldv, work, bstride, st.nst, p.n3, st.d_off,
+                         st.d_gpos, st.group, p.line[2], twiddles<T>(p, 2));
     const int per = (nbc + kDensitySlices - 1) / kDensitySlices;
     const dim3 grid(p.n3, kDensitySlices);
     const int acc = b0 > 0 ? 1 : 0;
@@ -1442,7 +1455,7 @@ int accumulate_density_t(const Plan& p, const void* vectors, int64_t ldv, int nb
     hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density_team<T, NN, PR>),          \
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)team_lds);          \
     hipLaunchKernelGGL((k_density_team<T, NN, PR>), grid, dim3(kTeamThreads), team_lds,     \
-                       stream, work, bstride, st.nst, st.d_stickq, st.d_colmap, st.nxcols,  \
+                       stream, work, bstride, st.nst, st.d_colpos, st.d_colmap, st.nxcols,  \
                        nbc, per, weights + b0, partial, acc, twiddles<T>(p, 0));             \
   } while (0)
 #define QEPW_LAUNCH_DENSITY(NN)                                                              \
