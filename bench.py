@@ -23,11 +23,12 @@ The JSON line carries:
   reference) on the same inputs and kwargs, on <= 16 host threads, for a
   bounded time budget; if it does not finish, its time is extrapolated by
   the H-application count of the device solve.
-* ``density``, ``projectors``, ``iterative`` and ``wavefunction_io`` (unless
-  ``--no-legs``): the SCF's density accumulation, the per-k projector
-  assembly, the CG / ParO / RMM-DIIS solvers and the wavefunction working
-  file on the same workload, each with its oracle timed on the host beside
-  it (unless ``--no-cpu``).
+* ``density``, ``projectors``, ``iterative``, ``wavefunction_io`` and
+  ``gradients`` (unless ``--no-legs``): the SCF's density accumulation, the
+  per-k projector assembly, the CG / ParO / RMM-DIIS solvers, the
+  wavefunction working file and the band energy's backward (dE/dV, dE/dpsi*,
+  dE/dtau) on the same workload, each with its oracle timed on the host
+  beside it (unless ``--no-cpu``).
 """
 
 from __future__ import annotations
@@ -665,6 +666,74 @@ def iterative_cpu_baseline(system, trial, device_result, device_ms):
             device_result.hamiltonian_applications}
 
 
+def gradients_leg(system, device):
+    """The differentiation workload: the band energy E = sum_n f_n <psi_n|H|psi_n>
+    of the medium system's 128 lowest trial bands (f = 2, 72^3 grid, 64 atoms)
+    and its backward -- dE/dV(r), dE/dpsi*, dE/dtau with the projectors
+    recomputed from tau -- through the torch.autograd.Function, complex128 and
+    complex64 storage.  One step = forward + backward with host inputs
+    resident on the device, timed on the wall clock around a device sync
+    (the forward reads the energy back), median of 5 after a warm-up."""
+    import numpy as np
+    import torch
+
+    from qe_python_amd.gradients import BandEnergyModel, band_energy
+
+    nb, reps = 128, 5
+    psi = system.trial[:, :nb]
+    f = np.full(nb, 2.0)
+    model = BandEnergyModel(system.basis, system.shape, system.projectors.beta_species,
+                            system.projectors.couplings, device=device)
+    p = torch.tensor(psi, device=device, requires_grad=True)
+    v = torch.tensor(system.real_potential, device=device, requires_grad=True)
+    t = torch.tensor(system.projectors.positions, device=device, requires_grad=True)
+    rec = {"workload": "medium: E and dE/dV, dE/dpsi*, dE/dtau of 128 bands, f = 2, "
+                       "72^3 grid, 64 atoms / 256 projectors"}
+    outputs = {}
+    for precision in ("fp64", "fp32"):
+        times = []
+        for i in range(reps + 1):
+            for x in (p, v, t):
+                x.grad = None
+            torch.cuda.synchronize(device)
+            t0 = time.perf_counter()
+            e = band_energy(model, p, v, t, f, precision=precision)
+            e.backward()
+            torch.cuda.synchronize(device)
+            if i:
+                times.append(1e3 * (time.perf_counter() - t0))
+        rec[precision] = {"ms_per_step": float(np.median(times)), "ms_each": times}
+        outputs[precision] = (float(e.detach()), v.grad.cpu().numpy(), t.grad.cpu().numpy(),
+                              p.grad.cpu().numpy())
+    return rec, (psi, f), outputs
+
+
+def gradients_cpu_baseline(system, inputs, outputs, device_ms):
+    """The oracle's restatement of E and its three gradients on the host, same
+    bands and occupations, and the device results checked against it."""
+    import numpy as np
+
+    from oracle.forces import band_energy_gradients
+
+    psi, f = inputs
+    t0 = time.perf_counter()
+    e, dv, dtau, gpsi = band_energy_gradients(
+        system.basis.indices, system.basis.kinetic, system.basis.vectors, system.shape,
+        system.real_potential, system.projectors.beta_species, system.projectors.couplings,
+        system.projectors.positions, psi, f)
+    cpu_ms = 1e3 * (time.perf_counter() - t0)
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    out = {"cpu_ms_oracle": cpu_ms, "gpu_over_cpu": cpu_ms / device_ms}
+    for precision, (e_d, dv_d, dtau_d, gpsi_d) in outputs.items():
+        out[f"errors_{precision}"] = {"energy": abs(e_d - e) / abs(e), "dE_dV": rel(dv_d, dv),
+                                      "dE_dtau": rel(dtau_d, dtau),
+                                      "dE_dpsi": rel(gpsi_d, gpsi)}
+    return out
+
+
 def wavefunction_io_leg(system, device, with_cpu):
     """The SCF's wavefunction working file (WavefunctionBuffer,
     pw/buffers.py:109-232) with device eigenvectors: one medium record (128
@@ -847,6 +916,8 @@ def main():
                                               torch.from_numpy(system.trial).to(device))
         line["iterative"] = iterative
         line["wavefunction_io"] = wavefunction_io_leg(system, device, not args.no_cpu)
+        gradients, g_inputs, g_outputs = gradients_leg(system, device)
+        line["gradients"] = gradients
     if not args.no_cpu:
         happ = head["hamiltonian_applications"][-1]
         line["cpu_baseline"] = cpu_baseline(system, kwargs, happ, args.cpu_budget,
@@ -858,6 +929,8 @@ def main():
                                                              projectors["ms_per_call"]))
             line["iterative"]["rmm_diis"].update(iterative_cpu_baseline(
                 system, system.trial, rmm_result, iterative["rmm_diis"]["ms_per_call"]))
+            line["gradients"].update(gradients_cpu_baseline(
+                system, g_inputs, g_outputs, gradients["fp64"]["ms_per_step"]))
         if head["value"] > 0:
             line["config"]["gpu_over_cpu"] = head["value"] / line["cpu_baseline"]["value"]
     print(json.dumps(line))
