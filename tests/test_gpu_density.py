@@ -15,7 +15,8 @@ from qe_python_amd.synthetic import make_system
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name,nb", [("small", 16), ("medium", 24)])
+@pytest.mark.parametrize("name,nb", [("small", 16), ("medium", 24), ("cell54", 16),
+                                     ("cell64", 16)])
 def test_density_matches_oracle(name, nb):
     from qe_python_amd.basis import LocalPotentialWorkspace
     s = make_system(name)
