@@ -458,29 +458,32 @@ class DeviceRayleighRitz:
             return None
         sigma, lo_v, hi_v = shift
         lib = nat.load()
-        have_prev = self.u_prev is not None
-        if not have_prev:
-            self.u_prev = torch.empty((k, self.maxsub), dtype=C128, device=self.device)
         coef = torch.empty((k, n), dtype=C128, device=self.device)
         buf = self.ws.get(int(lib.qepw_rr_workspace(self.maxsub)))
         args = nat.RRSpectralArgs(
             n=n, k=k, maxsub=self.maxsub, count_hint=len(self.theta) + 24,
-            host_single=int(self.host_single), have_prev=int(have_prev), sigma=sigma,
+            host_single=int(self.host_single), sigma=sigma,
             gap=hi_v - lo_v, bound_prev=self._bound_prev or 0.0, A=self.A.data_ptr(),
             W=self.W.data_ptr(), X=self.X.data_ptr(), X2=self.X2.data_ptr(),
-            Y=self.Y.data_ptr(), u_prev=self.u_prev.data_ptr(),
-            gauss=_gaussian_columns(self.maxsub, self.device).data_ptr(),
+            Y=self.Y.data_ptr(), gauss=_gaussian_columns(self.maxsub, self.device).data_ptr(),
             coef=coef.data_ptr(), rb=self.rb.data_ptr(), work=buf.data_ptr(),
             work_bytes=buf.numel(), stream=dv.stream_handle(self.device))
         # buf stays referenced with the arguments that point into it
-        return args, coef, sigma, have_prev, buf
+        return args, coef, sigma, buf
 
     def _spectral_native_run(self, prepared):
         if prepared is None:
             return None
-        args, coef, sigma, have_prev, _ = prepared
+        args, coef, sigma, _ = prepared
         k = self.nroots
         lib = nat.load()
+        # the previous eigenvectors (allocated here, after the standardisation
+        # has succeeded, so that a failed one leaves no unset block behind)
+        have_prev = self.u_prev is not None
+        if not have_prev:
+            self.u_prev = torch.empty((k, self.maxsub), dtype=C128, device=self.device)
+        args.have_prev = int(have_prev)
+        args.u_prev = self.u_prev.data_ptr()
         values = np.empty(self.maxsub, dtype=np.float64)
         res = nat.RRSpectralResult()
         nat.check(lib.qepw_rr_spectral(ctypes.byref(args), values.ctypes.data,
