@@ -89,7 +89,22 @@ class _Blocks:
         return dv.upload(np.array(values, dtype=np.float64).reshape(-1), self.device)
 
     def index(self, positions) -> torch.Tensor:
-        return torch.as_tensor(np.asarray(positions, dtype=np.int64), device=self.device)
+        """Host positions -> device int32 map for ``take`` / ``put``."""
+        return dv.index_tensor(positions, self.device)
+
+    def take(self, block: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
+        """The columns idx of a (nb, npw) block, gathered into a new block
+        (qepw_copy_columns)."""
+        count = int(idx.shape[0])
+        out = torch.empty((count, block.shape[1]), dtype=block.dtype, device=self.device)
+        if count:
+            dv.copy_columns(block, out, self.npw, count, src_cols=idx)
+        return out
+
+    def put(self, dst: torch.Tensor, idx: torch.Tensor, src: torch.Tensor) -> None:
+        """dst[idx[j]] = src[j] (qepw_copy_columns)."""
+        if int(idx.shape[0]):
+            dv.copy_columns(src, dst, self.npw, int(idx.shape[0]), dst_cols=idx)
 
     # -- algebra ------------------------------------------------------------
     def apply(self, src: torch.Tensor) -> torch.Tensor:
@@ -310,7 +325,7 @@ def conjugate_gradient(operator, diagonal, number_of_roots, *, initial_vectors,
     order = np.argsort(values)
     values = values[order]
     idx = s.index(order)
-    v, hv = v.index_select(0, idx), hv.index_select(0, idx)
+    v, hv = s.take(v, idx), s.take(hv, idx)
     return DavidsonResult(values, s.result(v, device_output), s.residual_norms(hv, v, values),
                           max(1, int(round(iteration_sum / number_of_roots))), unconverged == 0,
                           unconverged, s.happ)
@@ -347,11 +362,9 @@ def _paro_bpcg(s: _Blocks, reference, x0, hx0, energies, tolerance, max_iteratio
         ia = None if full else s.index(active)
 
         def take(block):
-            return block if full else block.index_select(0, ia)
+            return block if full else s.take(block, ia)
 
         d = take(dirs)
-        if not full:
-            d = d.contiguous()
         hd = s.apply(d)
         e = e_all[active]
         dd, dhd = s.dots((d, d), (d, hd))
@@ -378,8 +391,8 @@ def _paro_bpcg(s: _Blocks, reference, x0, hx0, energies, tolerance, max_iteratio
             s.axpby(d, undo, c_act, None)
             s.axpby(hd, undo, hc_act, None)
         if not full:
-            corr.index_copy_(0, ia, c_act)
-            hcorr.index_copy_(0, ia, hc_act)
+            s.put(corr, ia, c_act)
+            s.put(hcorr, ia, hc_act)
         done = (np.abs(g1) < thresholds[active]) | rose | (it == max_iterations)
         keep = ~done
         if not np.any(keep):
@@ -387,12 +400,12 @@ def _paro_bpcg(s: _Blocks, reference, x0, hx0, energies, tolerance, max_iteratio
         cont = active[keep]
         beta = (g1[keep] - g2[keep]) / _guard(g0[cont])
         ik = s.index(np.flatnonzero(keep))
-        d_keep = d.index_select(0, ik)
-        new_keep = newp.index_select(0, ik)
+        d_keep = s.take(d, ik)
+        new_keep = s.take(newp, ik)
         s.axpby(new_keep, None, d_keep, beta)
         ic = s.index(cont)
-        dirs.index_copy_(0, ic, d_keep)
-        prec.index_copy_(0, ic, new_keep)
+        s.put(dirs, ic, d_keep)
+        s.put(prec, ic, new_keep)
         g0[cont] = g1[keep]
         f_prev[cont] = fval[keep]
         active = cont
@@ -449,8 +462,7 @@ def parallel_orbital(operator, diagonal, number_of_roots, *, initial_vectors,
                     break
         cols = cols[:nact]
         ic = s.index(cols)
-        corr, hcorr = _paro_bpcg(s, basis[:n], basis.index_select(0, ic),
-                                 hbasis.index_select(0, ic),
+        corr, hcorr = _paro_bpcg(s, basis[:n], s.take(basis, ic), s.take(hbasis, ic),
                                  np.asarray(energies[:nact], dtype=float), tolerance)
         dim = base + nact
         basis[base:dim] = corr
@@ -550,14 +562,15 @@ def rmm_diis(operator, diagonal, number_of_roots, *, initial_vectors, tolerance=
             break
         na = len(active)
         ia = s.index(active)
-        ia32 = ia.to(torch.int32)
-        vh[idiis].index_copy_(0, ia, v.index_select(0, ia))
-        hh[idiis].index_copy_(0, ia, hv.index_select(0, ia))
+        # history slot idiis of the active bands (gather and scatter by the
+        # same map)
+        dv.copy_columns(v, vh[idiis], npw, na, src_cols=ia, dst_cols=ia)
+        dv.copy_columns(hv, hh[idiis], npw, na, src_cols=ia, dst_cols=ia)
         eh[active, idiis] = values[active]
         cnt = idiis + 1
         if cnt == 1:
             # v, hv already hold the stored history: normalise in place
-            va, hva = v.index_select(0, ia), hv.index_select(0, ia)
+            va, hva = s.take(v, ia), s.take(hv, ia)
             (nn,) = s.dots((va, va))
             inv = 1.0 / np.sqrt(np.maximum(np.real(nn), 1.0e-300))
             s.scale(va, cfac=inv, dst=va)
@@ -565,8 +578,11 @@ def rmm_diis(operator, diagonal, number_of_roots, *, initial_vectors, tolerance=
             ra = hva.clone()
             dv.subtract_band_energies(ra, va, s.col(eh[active, 0]), npw)
         else:
-            hist_v = vh[:cnt].index_select(1, ia)      # (cnt, na, npw)
-            hist_h = hh[:cnt].index_select(1, ia)
+            # the active bands' history, (cnt, na, npw): rows c n + band of
+            # the (nd n, npw) history stores
+            ih = s.index((np.arange(cnt)[:, None] * n + active[None, :]).reshape(-1))
+            hist_v = s.take(vh.view(nd * n, npw), ih).view(cnt, na, npw)
+            hist_h = s.take(hh.view(nd * n, npw), ih).view(cnt, na, npw)
             pairs = []
             for i in range(cnt):
                 for j in range(cnt):
@@ -600,9 +616,9 @@ def rmm_diis(operator, diagonal, number_of_roots, *, initial_vectors, tolerance=
             va = torch.empty((na, npw), dtype=C128, device=s.device)
             hva = torch.empty_like(va)
             ra = torch.empty_like(va)
-            dv.history_combine(vh, cnt, ia32, cdev, va, npw)
-            dv.history_combine(hh, cnt, ia32, cdev, hva, npw)
-            dv.history_combine(vh, cnt, ia32, cedev, ra, npw)
+            dv.history_combine(vh, cnt, ia, cdev, va, npw)
+            dv.history_combine(hh, cnt, ia, cdev, hva, npw)
+            dv.history_combine(vh, cnt, ia, cedev, ra, npw)
             s.axpby(hva, None, ra, -1.0)                   # H phi c - phi (e c)
         # Kresse-Furthmueller correction and its H
         ekin = np.maximum(dv.column_diagonal_expectations(va, t_ry, npw).cpu().numpy(), 1.0e-14)
@@ -623,8 +639,8 @@ def rmm_diis(operator, diagonal, number_of_roots, *, initial_vectors, tolerance=
         nrm = np.sqrt(psp + 2.0 * ksp * step + ksk * step * step)
         s.axpby(kc, step / nrm, va, 1.0 / nrm)
         s.axpby(hkc, step / nrm, hva, 1.0 / nrm)
-        v.index_copy_(0, ia, va)
-        hv.index_copy_(0, ia, hva)
+        s.put(v, ia, va)
+        s.put(hv, ia, hva)
         old = values.copy()
         values = np.real(s.dots((v, hv))[0])
         res = hv.clone()
