@@ -153,24 +153,27 @@ class _BandEnergy(torch.autograd.Function):
         tau = positions.detach().to(device=device, dtype=torch.float64).contiguous()
         proj = model.projectors(tau, dtype)
         hpsi, overlap, coupled = model.apply(psi_dev, v_dev, proj)
-        f = torch.as_tensor(occupations, dtype=torch.float64).to(device)
+        f = np.asarray(occupations, dtype=np.float64).reshape(-1)
         ip = dv.column_inner_products(psi_dev, hpsi, model.npw)
-        energy = float(np.dot(f.cpu().numpy(), ip.cpu().numpy()[0::2]))
+        energy = float(np.dot(f, ip.cpu().numpy()[0::2]))
         ctx.model = model
-        ctx.save_for_backward(psi_dev, hpsi, proj, coupled, f)
+        ctx.save_for_backward(psi_dev, hpsi, proj, coupled)
+        ctx.occupations = f
         ctx.psi_dtype = psi.dtype
         return torch.tensor(energy, dtype=torch.float64, device=device)
 
     @staticmethod
     def backward(ctx, grad_energy):
         model = ctx.model
-        psi_dev, hpsi, proj, coupled, f = ctx.saved_tensors
+        psi_dev, hpsi, proj, coupled = ctx.saved_tensors
         g = float(grad_energy)
-        wf = f * g
+        # the per-band weights f_n dE are host scalars, uploaded once
+        wf_host = ctx.occupations * g
+        wf = dv.upload(wf_host, model.device)
         grad_psi = grad_v = grad_tau = None
         if ctx.needs_input_grad[0]:
             scaled = torch.empty_like(hpsi)
-            model.scale_rows(hpsi, scaled, cfac=2.0 * wf)
+            model.scale_rows(hpsi, scaled, cfac=dv.upload(2.0 * wf_host, model.device))
             ref = torch.empty_like(scaled)
             dv.permute_rows(1, model.plan.perm_device, scaled, ref, model.npw)
             grad_psi = ref.T.to(ctx.psi_dtype)

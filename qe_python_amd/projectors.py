@@ -53,7 +53,11 @@ def projector_basis(gk, table, dq: float, channels, device=None) -> torch.Tensor
                        or np.any(chans[:, 2] < 0) or np.any(chans[:, 2] > 2 * chans[:, 1])):
         raise ValueError("channels must be (radial index, l <= 3, 0 <= m <= 2l)")
     npw = int(q.shape[0])
-    qmax = float(torch.linalg.vector_norm(q, dim=1).max()) if npw else 0.0
+    # the table must cover every |k+G| (the kernel's interpolation reads four
+    # points from floor(|q| / dq)): checked on the host copy of the vectors
+    host_q = (gk.detach().to("cpu", torch.float64).numpy() if isinstance(gk, torch.Tensor)
+              else np.asarray(gk, dtype=np.float64))
+    qmax = float(np.sqrt(np.max(np.sum(host_q * host_q, axis=1)))) if npw else 0.0
     if npw and int(np.floor(qmax / dq)) + 3 >= nq:
         raise ValueError(f"the radial table (q up to {(nq - 1) * dq:.4g}) does not cover "
                          f"|k+G| = {qmax:.4g} with the four-point interpolation")
