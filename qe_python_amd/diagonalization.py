@@ -614,12 +614,13 @@ def _host_dtype(dtype: torch.dtype):
 
 
 def _cholesky_orthonormalize(block: torch.Tensor, npw: int, ws, condition: float = 1.0e6):
-    """(steps) Cholesky QR of a block of complex64 corrections: Gram = L L^H on the
+    """(steps) Cholesky QR of a block (the complex64 phase's corrections, or a
+    Davidson's initial block in either precision): Gram = L L^H in FP64 on the
     host (one thread), block L^-H on the device.  None when the Gram matrix
-    is not safely positive definite at FP32 precision (its Cholesky fails or
-    the squared condition estimate max(diag L)^2 / min(diag L)^2 exceeds
-    ``condition``); the caller then drops dependent directions with the
-    canonical orthonormalisation."""
+    is not safely positive definite at the block's precision (its Cholesky
+    fails or the squared condition estimate max(diag L)^2 / min(diag L)^2
+    exceeds ``condition``); the caller then drops dependent directions with
+    the canonical orthonormalisation."""
     n = int(block.shape[0])
     device = block.device
     if NATIVE_ORTHO and may_block():
