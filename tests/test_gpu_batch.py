@@ -57,13 +57,13 @@ def sequential(systems):
                      operator_into=h.apply_into, **KWARGS) for _, s, h in systems]
 
 
-@pytest.mark.parametrize("schedule", ["rounds", "threads"])
-def test_batch_matches_oracle_and_sequential(systems, sequential, schedule):
+@pytest.mark.parametrize("schedule,workers", [("rounds", 4), ("threads", 4), ("threads", None)])
+def test_batch_matches_oracle_and_sequential(systems, sequential, schedule, workers):
     from qe_python_amd.batch import davidson_batch
     from qe_python_amd.schedule import RoundStats
     golden = np.load(GOLDEN)
     stats = RoundStats()
-    results = davidson_batch(batch_problems(systems), workers=4, schedule=schedule,
+    results = davidson_batch(batch_problems(systems), workers=workers, schedule=schedule,
                              stats=stats, **KWARGS)
     if schedule == "rounds":
         # every round groups its H applications by precision: at most two
