@@ -129,7 +129,8 @@ def test_non_square_plane_generic_kernel():
 
 
 def test_team_kernel_sizes_against_oracle():
-    """Square planes of several team-FFT lengths (factorisations 4x6 ... 9x10)."""
+    """Square planes of several team-FFT lengths (factorisations 4x6 ... 9x10),
+    in FP64 and FP32 (each its own kernel instantiation)."""
     from qe_python_amd.basis import make_bases
     from qe_python_amd.diagonalization import PlaneWaveHamiltonian
     rng = np.random.default_rng(23)
@@ -144,6 +145,9 @@ def test_team_kernel_sizes_against_oracle():
         ham = PlaneWaveHamiltonian(basis, np.zeros(shape, complex), (), real_potential=vr)
         ref = OracleHamiltonian(basis.indices, basis.kinetic, shape, vr).apply(x)
         assert rel(ham.apply(x), ref) <= 1e-12, n
+        ham32 = PlaneWaveHamiltonian(basis, np.zeros(shape, complex), (), real_potential=vr,
+                                     precision="fp32")
+        assert rel(ham32.apply(x), ref) <= 2e-6, n
 
 
 def test_square_planes_without_team_kernel_use_the_generic_kernel():
